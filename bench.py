@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""HPCG solver benchmark on one MI355X.
+
+A step is one preconditioned CG iteration (4-level MG V-cycle with one pre-
+and one post-smoothing multicolour SymGS per level, outer SpMV, dots and
+vector updates) on the 27-point problem, all arrays resident in HBM.
+
+Prints one JSON line: the FP64 HPCG-formula GFLOP/s (value), the mixed-
+precision CG's GFLOP/s and its time to a relative residual of 1e-9 against
+the FP64 CG's, the modelled HBM TB/s of both, a roofline entry for the outer
+SpMV kernel timed with HIP events on the solver stream, and the CPU baseline
+(the oracle's CG_ref restatement, OpenMP, on this host's cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "HPCG GFLOP/s + HBM TB/s: mixed vs FP64 CG to rel. residual 1e-9, 1 GPU"
+HBM_PEAK_GBPS = 8000.0
+
+
+def level_rows(dims, levels):
+    out = []
+    d = list(dims)
+    for _ in range(levels):
+        out.append((d[0] * d[1] * d[2], (3 * d[0] - 2) * (3 * d[1] - 2) * (3 * d[2] - 2)))
+        d = [v // 2 for v in d]
+    return out
+
+
+def bytes_per_iteration(dims, levels, mode):
+    """Algorithmic HBM bytes of one CG iteration of this implementation:
+    every ELL slot read once per pass (27 x (4 B column + value)), vectors
+    read/written once, neighbour gathers assumed served from cache."""
+    rows = level_rows(dims, levels)
+    m0 = rows[0][0]
+    if mode == "f64":
+        vb, xb, rb = 8, 8, 8
+    else:
+        vb, xb, rb = 2, 4, 8
+    slot = 4 + vb
+    total = 0.0
+    for lv, (m, _) in enumerate(rows):
+        r_bytes = rb if lv == 0 else xb
+        zero = 27 * slot + r_bytes + xb + 7 / 8 * 2 * xb
+        if lv + 1 < levels:
+            restrict = (27 * slot + r_bytes + 8 + xb) / 8
+            prolong = (8 + 3 * xb) / 8
+            post = (27 * slot + r_bytes + xb + 7 / 8 * xb) + 7 / 8 * (4 + vb + 2 * xb + xb) + 13 * slot
+            total += m * (zero + restrict + prolong + post)
+        else:
+            total += m * zero
+    spmv = 27 * (4 + vb) + 16
+    zb = 8 if mode == "f64" else 4
+    vec = (8 + zb) + (zb + 16) + 48
+    total += m0 * (spmv + vec)
+    if mode == "mixed":
+        total += m0 * (spmv + 24 + 8) / 10.0   # true residual every 10 iterations
+    return total
+
+
+def spmv_bytes(m, vb=8):
+    return m * (27 * (4 + vb) + 8 + 8)
+
+
+def cpu_baseline(dims, iters, threads):
+    import oracle
+    oracle.set_threads(threads)
+    t0 = time.perf_counter()
+    P = oracle.Problem(*dims, nlevels=4, order=oracle.ORDER_NATURAL)
+    setup = time.perf_counter() - t0
+    b = P.rhs_ones()
+    t0 = time.perf_counter()
+    res = P.cg(b, max_iter=iters, tol=0.0)
+    el = time.perf_counter() - t0
+    flops = oracle.hpcg_flops_per_iter(dims) * res["niters"]
+    P.close()
+    return dict(value=round(flops / el / 1e9, 3), unit="GFLOP/s", cores=threads, kind="port",
+                sample=f"oracle CG_ref restatement (natural order, OpenMP except SymGS) at "
+                       f"{dims[0]}x{dims[1]}x{dims[2]}, {res['niters']} iterations, "
+                       f"{el:.1f} s timed ({setup:.1f} s setup untimed)",
+                s_per_iter=round(el / max(res["niters"], 1), 3))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=320)
+    ap.add_argument("--ny", type=int, default=None)
+    ap.add_argument("--nz", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-solve", action="store_true", help="skip the solve-to-1e-9 comparison")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of a hipGraph")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs (--gpus > 1) "
+                                   "are not part of it", "n_gpus": args.gpus}))
+        sys.exit(2)
+    dims = (args.nx, args.ny or args.nx, args.nz or args.nx)
+    import rochpcg_amd as hip
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    G = hip.Problem(*dims, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
+    log(f"setup {dims} on the GPU: {time.perf_counter() - t0:.2f} s")
+    flops = G.flops_per_iteration()
+    K, W = args.steps, args.warmup
+    use_graph = not args.eager
+
+    def timed(mode):
+        G.cg_begin(mode=mode, max_iter=W + K, tol=0.0)
+        G.cg_iterate(W, use_graph=use_graph)
+        G.synchronize()
+        t = time.perf_counter()
+        G.cg_iterate(K, use_graph=use_graph)
+        G.synchronize()
+        el = time.perf_counter() - t
+        res = G.cg_end()
+        assert res["niters"] == W + K, res
+        return el
+
+    import numpy as np
+    G.vec["x"].upload(np.zeros(G.m))
+    el64 = timed(hip.CG_F64)
+    log(f"FP64: {K} iterations in {el64 * 1e3:.1f} ms")
+    G.vec["x"].upload(np.zeros(G.m))
+    elmx = timed(hip.CG_MIXED)
+    log(f"mixed: {K} iterations in {elmx * 1e3:.1f} ms")
+    ms64 = el64 * 1e3 / K
+    msmx = elmx * 1e3 / K
+    b64 = bytes_per_iteration(dims, 4, "f64")
+    bmx = bytes_per_iteration(dims, 4, "mixed")
+
+    # per-phase HIP-event timing of K FP64 iterations (eager launches)
+    G.vec["x"].upload(np.zeros(G.m))
+    G.cg_begin(mode=hip.CG_F64, max_iter=K, tol=0.0)
+    ph = G.cg_iterate_profiled(K)
+    G.cg_end()
+    spmv_ms = ph["spmv"] / K
+    spmv_gbps = spmv_bytes(G.m) / (spmv_ms * 1e-3) / 1e9
+    log(f"phases (ms per iteration): " + ", ".join(f"{k} {v / K:.3f}" for k, v in ph.items()))
+
+    out = {
+        "metric": METRIC,
+        "value": round(flops / (ms64 * 1e-3) / 1e9, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(ms64, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"HPCG 27-point {dims[0]}x{dims[1]}x{dims[2]}, 4-level MG "
+                               f"(1 pre/1 post multicolour SymGS), FP64 CG, b=A*ones, x0=0",
+                   "nx": dims[0], "ny": dims[1], "nz": dims[2], "levels": 4,
+                   "graph": use_graph, "parallelism": "single GPU"},
+        "hbm_TBps": round(b64 / (ms64 * 1e-3) / 1e12, 3),
+        "bytes_per_iteration": int(b64),
+        "mixed": {"value": round(flops / (msmx * 1e-3) / 1e9, 2), "unit": "GFLOP/s",
+                  "ms_per_step": round(msmx, 4), "hbm_TBps": round(bmx / (msmx * 1e-3) / 1e12, 3),
+                  "bytes_per_iteration": int(bmx), "speedup_per_iteration": round(ms64 / msmx, 3),
+                  "dtype": "f64 outer CG / f32 vectors + f16 ELL in MG"},
+        "phase_ms_per_iteration": {k: round(v / K, 4) for k, v in ph.items()},
+        "roofline": {"kernel": "k_spmv (outer SpMV + fused p'Ap), level 0",
+                     "bound": "hbm", "achieved": round(spmv_gbps, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(spmv_gbps / HBM_PEAK_GBPS, 4),
+                     "traffic": None, "bytes_per_launch": spmv_bytes(G.m),
+                     "avg_launch_ms": round(spmv_ms, 4)},
+    }
+
+    if not args.no_solve:
+        # solve to 1e-9 from x0 = 0 with x* uniform in [-1,1] (splitmix64, seed 1234)
+        G.make_rhs(1, 1234)
+        xs = G.vec["xexact"].host()
+        sol = {}
+        for name, mode in (("f64", hip.CG_F64), ("mixed", hip.CG_MIXED)):
+            G.vec["x"].upload(np.zeros(G.m))
+            G.synchronize()
+            t = time.perf_counter()
+            r = G.cg(max_iter=500, tol=1e-9, mode=mode, use_graph=use_graph, history=False)
+            el = time.perf_counter() - t
+            x = G.vec["x"].host()
+            # true residual in f64 on the GPU: r = b - A x
+            d = hip.DeviceArray(G.m)
+            G.spmv(G.vec["x"], d)
+            G.waxpby(1.0, G.vec["b"], -1.0, d, d)
+            rt = np.sqrt(G.dot(d, d)) / np.sqrt(G.dot(G.vec["b"], G.vec["b"]))
+            sol[name] = {"iterations": r["niters"], "seconds": round(el, 4),
+                         "rel_residual": float(f"{r['normr'] / r['normr0']:.3e}"),
+                         "true_rel_residual": float(f"{rt:.3e}"),
+                         "err_vs_xstar": float(f"{np.linalg.norm(x - xs) / np.linalg.norm(xs):.3e}"),
+                         "gflops": round(flops * r["niters"] / el / 1e9, 2)}
+            log(f"solve {name}: {sol[name]}")
+        sol["speedup_time_to_solution"] = round(sol["f64"]["seconds"] / sol["mixed"]["seconds"], 3)
+        sol["iteration_delta"] = sol["mixed"]["iterations"] - sol["f64"]["iterations"]
+        out["solve_1e-9"] = sol
+    G.close()
+
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"CPU baseline on {threads} threads ...")
+        out["cpu_baseline"] = cpu_baseline(dims, args.cpu_iters, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

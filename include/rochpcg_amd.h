@@ -1,0 +1,165 @@
+/*
+ * rochpcg_amd.h -- C ABI of the MI355X-native HPCG solver hot path.
+ *
+ * A drop-in for the optimized CG/MG path of ROCm/rocHPCG on one GPU, single
+ * rank.  Plain pointers and sizes only; every entry point returns 0 on success
+ * and a non-zero status otherwise (hpcgx_last_error() holds the message) --
+ * the library never calls exit(), unlike the reference's HIP_CHECK
+ * (utils.hpp:63-76).
+ *
+ * Conventions shared with the reference's optimized path:
+ *   - all device vectors are double precision and in multicolour row order
+ *     (the order OptimizeProblem leaves them in, OptimizeProblem.cpp:69-160);
+ *   - operators are asynchronous on the problem's stream; hpcgx_dot and
+ *     hpcgx_fused_waxpby_dot return a host double and therefore synchronise,
+ *     like ComputeDotProduct.cpp:216-217 and ComputeWAXPBY.cpp:218-219;
+ *   - calls on one problem are not re-entrant (each problem owns its scratch).
+ *
+ * Each declaration names the reference interface it replaces (file:line of
+ * ROCm/rocHPCG src/).
+ */
+#ifndef ROCHPCG_AMD_H
+#define ROCHPCG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hpcgx_problem hpcgx_problem;
+
+/* precision stores built for a problem (bit mask) */
+#define HPCGX_STORE_F64 1   /* f64 ELL values: the reference's arithmetic            */
+#define HPCGX_STORE_MIXED 2 /* f16 ELL values + f32 MG vectors (mixed-precision CG)   */
+
+/* CG modes */
+#define HPCGX_CG_F64 0      /* everything in f64                                      */
+#define HPCGX_CG_MIXED 1    /* f64 outer CG, f32/f16 V-cycle, true residual every k   */
+
+/* ---- setup ------------------------------------------------------------- */
+
+/* GenerateGeometry + GenerateProblem + SetupHalo (single rank) +
+ * GenerateCoarseProblem x (nlevels-1) + OptimizeProblem, all on the device:
+ * GenerateProblem.cpp:375-489, GenerateCoarseProblem.cpp:98-184,
+ * SetupHalo.cpp:313-317, OptimizeProblem.cpp:69-160 (JPLColoring,
+ * PermuteColumns, ConvertToELL, PermuteRows, ExtractDiagonal).  The problem's
+ * b = A*ones, x = 0, xexact = 1 are ready in multicolour order.
+ * stream: a hipStream_t to run on, or NULL for a private non-blocking stream. */
+int hpcgx_problem_create(int nx, int ny, int nz, int nlevels, int stores, void* stream,
+                         hpcgx_problem** out);
+/* DeleteMatrix / DeleteMGData / HIPDeleteCGData (SparseMatrix.hpp:264-330,
+ * MGData.hpp:101-120, CGData.hpp:99-105) */
+int hpcgx_problem_destroy(hpcgx_problem* p);
+
+/* level geometry: info[0..7] = nx, ny, nz, rows, nonzeros, ncolors, coarse rows,
+ * stores (SparseMatrix fields localNumberOfRows, totalNumberOfNonzeros,
+ * nblocks; SparseMatrix.hpp:75-156) */
+int hpcgx_level_info(const hpcgx_problem* p, int level, int64_t* info);
+/* colour table: sizes[c], offsets[c] (c < ncolors, offsets has ncolors+1
+ * entries) and the per-JPL-round counts the reference records in A.sizes
+ * (MultiColoring.cpp:287-306) */
+int hpcgx_color_table(const hpcgx_problem* p, int level, int* sizes, int* offsets, int* round_sizes);
+/* device pointers of a level's arrays (SparseMatrix::ell_col_ind, ell_val,
+ * diag_idx, inv_diag, perm; MGData::d_f2cOperator); any output may be NULL.
+ * ptrs[0..8] = ell_col_ind, ell_val (f64 or NULL), ell_val_f16 (or NULL),
+ * diag_idx, inv_diag, perm (natural -> multicolour), colors (natural),
+ * f2c (coarse natural -> fine natural), f2c_fine (coarse natural -> fine
+ * multicolour row), f2c_coarse (coarse natural -> coarse multicolour row) */
+int hpcgx_level_arrays(const hpcgx_problem* p, int level, void** ptrs);
+/* device pointers of the CG data and system: ptrs[0..7] = b, x, xexact, r,
+ * z (f64, or NULL), p, Ap, z (f32 mixed-mode, or NULL)  (CGData.hpp:55-61) */
+int hpcgx_vectors(const hpcgx_problem* p, void** ptrs);
+/* refill b / xexact / x: kind 0 = the reference's b = A*ones, xexact = 1
+ * (GenerateProblem.cpp:300); kind 1 = xexact uniform in [-1,1] from
+ * splitmix64(seed), b = A*xexact summed in natural order; x = 0 */
+int hpcgx_make_rhs(hpcgx_problem* p, int kind, uint64_t seed);
+
+/* ---- operators (f64 device vectors, multicolour order) ------------------- */
+
+/* ComputeSPMV(A, x, y)               ComputeSPMV.cpp:226-278 */
+int hpcgx_spmv(hpcgx_problem* p, int level, const double* x, double* y);
+/* ComputeSYMGS(A, r, x)              ComputeSYMGS.cpp:369-404 */
+int hpcgx_symgs(hpcgx_problem* p, int level, const double* r, double* x);
+/* ComputeSYMGSZeroGuess(A, r, x)     ComputeSYMGS.cpp:406-454 */
+int hpcgx_symgs_zero(hpcgx_problem* p, int level, const double* r, double* x);
+/* ComputeMG(A, r, x)                 ComputeMG.cpp:64-102 */
+int hpcgx_mg(hpcgx_problem* p, int level, const double* r, double* x);
+/* ComputeMG in the mixed store: r f64, x f32 (no reference counterpart; the
+ * mixed-precision V-cycle of the mixed CG mode) */
+int hpcgx_mg_mixed(hpcgx_problem* p, const double* r, float* x);
+/* ComputeFusedSpMVRestriction(A, rf, xf): rc = restriction of rf - A xf into
+ * the next level's rhs buffer (MGData::rc), ComputeRestriction.cpp:199-237;
+ * rc_out (may be NULL) receives the device pointer of that buffer */
+int hpcgx_restriction(hpcgx_problem* p, int level, const double* rf, const double* xf, double** rc_out);
+/* ComputeProlongation(Af, xf): xf += P xc with xc = the next level's
+ * solution buffer (MGData::xc), ComputeProlongation.cpp:72-86; xc_out (may be
+ * NULL) receives the device pointer of that buffer */
+int hpcgx_prolongation(hpcgx_problem* p, int level, double* xf, double** xc_out);
+/* ComputeDotProduct(n, x, y, result, ...)   ComputeDotProduct.cpp:189-238 */
+int hpcgx_dot(hpcgx_problem* p, int64_t n, const double* x, const double* y, double* result);
+/* ComputeWAXPBY(n, alpha, x, beta, y, w)    ComputeWAXPBY.cpp:111-135 */
+int hpcgx_waxpby(hpcgx_problem* p, int64_t n, double alpha, const double* x, double beta,
+                 const double* y, double* w);
+/* ComputeFusedWAXPBYDot(n, alpha, x, y, result): y += alpha x; result = y'y
+ * ComputeWAXPBY.cpp:198-240 */
+int hpcgx_fused_waxpby_dot(hpcgx_problem* p, int64_t n, double alpha, const double* x, double* y,
+                           double* result);
+/* HIPCopyMatrixDiagonal / HIPReplaceMatrixDiagonal (SparseMatrix.cpp:102-164),
+ * f64 store only */
+int hpcgx_copy_diagonal(hpcgx_problem* p, int level, double* diag);
+int hpcgx_replace_diagonal(hpcgx_problem* p, int level, const double* diag);
+/* TestCG's diagonal/rhs exaggeration (TestCG.cpp:67-95): d[i], b[i] *= 1e6,
+ * (i+2)*1e6 for i < 9 */
+int hpcgx_testcg_scale(hpcgx_problem* p, double* diag, double* b);
+
+/* ---- CG --------------------------------------------------------------- */
+
+/* CG(A, data, b, x, max_iter, tolerance, niters, normr, normr0, times,
+ *    doPreconditioning)  CG.cpp:104-200.
+ * b, x: device vectors (NULL = the problem's own b / x).  history (host,
+ * may be NULL) receives normr after iterations 0..niters (hist_len entries at
+ * most).  times (host, may be NULL): times[0] += wall seconds.
+ * use_graph: replay each iteration as a captured hipGraph. */
+int hpcgx_cg(hpcgx_problem* p, int mode, const double* b, double* x, int max_iter, double tolerance,
+             int precond, int use_graph, int* niters, double* normr, double* normr0,
+             double* history, int hist_len, double* times);
+/* the same loop in pieces, for timing: preamble; `count` iterations enqueued
+ * (no synchronisation); results */
+int hpcgx_cg_begin(hpcgx_problem* p, int mode, const double* b, double* x, int max_iter,
+                   double tolerance, int precond);
+int hpcgx_cg_iterate(hpcgx_problem* p, int count, int use_graph);
+/* `count` iterations with HIP events around each phase; phase_ms[0..4] =
+ * SpMV(+p'Ap), MG, r'z, vector updates, true-residual recomputation */
+int hpcgx_cg_iterate_profiled(hpcgx_problem* p, int count, double* phase_ms);
+int hpcgx_cg_end(hpcgx_problem* p, int* niters, double* normr, double* normr0, double* history,
+                 int hist_len);
+/* mixed mode: recompute the true residual every k iterations (0 = never) */
+int hpcgx_set_residual_replacement(hpcgx_problem* p, int every);
+
+/* HPCG flop count of one CG iteration (ReportResults.cpp:73-89) */
+double hpcgx_flops_per_iteration(const hpcgx_problem* p);
+
+/* ---- timing and memory helpers ----------------------------------------- */
+
+/* average device time (ms) of `reps` back-to-back launches of one operator
+ * on `level`, measured with HIP events on the problem's stream:
+ * op 0 = SpMV (f64), 1 = SymGS (f64), 2 = zero-guess SymGS (f64),
+ * 3 = SpMV (f16 values, f64 vectors), 4 = SymGS (mixed), 5 = MG (f64),
+ * 6 = MG (mixed) */
+int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int reps, double* avg_ms);
+int hpcgx_synchronize(hpcgx_problem* p);
+int hpcgx_malloc(void** ptr, size_t bytes);
+int hpcgx_free(void* ptr);
+int hpcgx_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int hpcgx_memcpy_d2h(void* dst, const void* src, size_t bytes);
+int hpcgx_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int hpcgx_device_count(int* count);
+const char* hpcgx_last_error(void);
+const char* hpcgx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ROCHPCG_AMD_H */

@@ -1,0 +1,340 @@
+"""rochpcg_amd -- MI355X-native (gfx950) HPCG solver hot path.
+
+Python host binding of ``librochpcg_amd.so`` (the C ABI declared in
+``include/rochpcg_amd.h``).  It mirrors the reference's optimized-path
+interface of ROCm/rocHPCG (``ComputeSPMV``, ``ComputeSYMGS``, ``ComputeMG``,
+``ComputeDotProduct``, ``ComputeWAXPBY``, ``CG``; src/ComputeSPMV.hpp:20 and
+siblings) on device buffers in multicolour row order.
+
+There is no CPU fallback: importing fails loudly when the HIP library is not
+built, and every call raises ``HPCGError`` when the library reports an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librochpcg_amd.so")
+
+STORE_F64 = 1
+STORE_MIXED = 2
+CG_F64 = 0
+CG_MIXED = 1
+
+# hpcgx_time_operator op codes
+OP_SPMV, OP_SYMGS, OP_SYMGS_ZERO, OP_SPMV_F16, OP_SYMGS_MIXED, OP_MG, OP_MG_MIXED = range(7)
+PHASES = ("spmv", "mg", "ddot", "waxpby", "true_residual")
+
+
+class HPCGError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(rochpcg_amd has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, i64, i32, f64 = C.c_void_p, C.c_int64, C.c_int, C.c_double
+    P = C.POINTER
+    sig = {
+        "hpcgx_problem_create": ([i32, i32, i32, i32, i32, vp, P(vp)], i32),
+        "hpcgx_problem_destroy": ([vp], i32),
+        "hpcgx_level_info": ([vp, i32, P(i64)], i32),
+        "hpcgx_color_table": ([vp, i32, vp, vp, vp], i32),
+        "hpcgx_level_arrays": ([vp, i32, P(vp)], i32),
+        "hpcgx_vectors": ([vp, P(vp)], i32),
+        "hpcgx_make_rhs": ([vp, i32, C.c_uint64], i32),
+        "hpcgx_spmv": ([vp, i32, vp, vp], i32),
+        "hpcgx_symgs": ([vp, i32, vp, vp], i32),
+        "hpcgx_symgs_zero": ([vp, i32, vp, vp], i32),
+        "hpcgx_mg": ([vp, i32, vp, vp], i32),
+        "hpcgx_mg_mixed": ([vp, vp, vp], i32),
+        "hpcgx_restriction": ([vp, i32, vp, vp, P(vp)], i32),
+        "hpcgx_prolongation": ([vp, i32, vp, P(vp)], i32),
+        "hpcgx_dot": ([vp, i64, vp, vp, P(f64)], i32),
+        "hpcgx_waxpby": ([vp, i64, f64, vp, f64, vp, vp], i32),
+        "hpcgx_fused_waxpby_dot": ([vp, i64, f64, vp, vp, P(f64)], i32),
+        "hpcgx_copy_diagonal": ([vp, i32, vp], i32),
+        "hpcgx_replace_diagonal": ([vp, i32, vp], i32),
+        "hpcgx_testcg_scale": ([vp, vp, vp], i32),
+        "hpcgx_cg": ([vp, i32, vp, vp, i32, f64, i32, i32, P(i32), P(f64), P(f64), vp, i32, vp], i32),
+        "hpcgx_cg_begin": ([vp, i32, vp, vp, i32, f64, i32], i32),
+        "hpcgx_cg_iterate": ([vp, i32, i32], i32),
+        "hpcgx_cg_iterate_profiled": ([vp, i32, vp], i32),
+        "hpcgx_cg_end": ([vp, P(i32), P(f64), P(f64), vp, i32], i32),
+        "hpcgx_set_residual_replacement": ([vp, i32], i32),
+        "hpcgx_flops_per_iteration": ([vp], f64),
+        "hpcgx_time_operator": ([vp, i32, i32, i32, P(f64)], i32),
+        "hpcgx_synchronize": ([vp], i32),
+        "hpcgx_malloc": ([P(vp), C.c_size_t], i32),
+        "hpcgx_free": ([vp], i32),
+        "hpcgx_memcpy_h2d": ([vp, vp, C.c_size_t], i32),
+        "hpcgx_memcpy_d2h": ([vp, vp, C.c_size_t], i32),
+        "hpcgx_memcpy_d2d": ([vp, vp, C.c_size_t], i32),
+        "hpcgx_device_count": ([P(i32)], i32),
+        "hpcgx_last_error": ([], C.c_char_p),
+        "hpcgx_version": ([], C.c_char_p),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    return L
+
+
+lib = _load()
+EXPORTED = tuple(n for n in dir(lib) if n.startswith("hpcgx_"))
+
+
+def _check(rc):
+    if rc != 0:
+        raise HPCGError(lib.hpcgx_last_error().decode() or f"status {rc}")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = lib.hpcgx_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class DeviceArray:
+    """A raw device allocation with numpy-typed host transfers."""
+
+    def __init__(self, n, dtype=np.float64, ptr=None, owner=True):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        if ptr is None:
+            p = C.c_void_p()
+            _check(lib.hpcgx_malloc(C.byref(p), max(self.nbytes, 8)))
+            self.ptr = p.value
+            self.owner = True
+        else:
+            self.ptr = int(ptr)
+            self.owner = owner
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or np.asarray(a).dtype)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.n:
+            raise ValueError(f"size mismatch: {a.size} != {self.n}")
+        _check(lib.hpcgx_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes))
+
+    def host(self):
+        out = np.empty(self.n, self.dtype)
+        _check(lib.hpcgx_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.owner and self.ptr:
+            lib.hpcgx_free(self.ptr)
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ptr if isinstance(a, DeviceArray) else int(a)
+
+
+class Level:
+    def __init__(self, info):
+        (self.nx, self.ny, self.nz, self.m, self.nnz, self.ncolors, self.mc,
+         self.stores) = [int(v) for v in info]
+
+
+class Problem:
+    """A multigrid HPCG hierarchy resident on the GPU (multicolour order).
+
+    Setup = GenerateProblem + GenerateCoarseProblem + OptimizeProblem of the
+    reference, done on the device.  b = A*ones, x = 0, xexact = 1 initially.
+    """
+
+    def __init__(self, nx, ny=None, nz=None, nlevels=4, stores=STORE_F64):
+        ny = nx if ny is None else ny
+        nz = nx if nz is None else nz
+        h = C.c_void_p()
+        _check(lib.hpcgx_problem_create(nx, ny, nz, nlevels, stores, None, C.byref(h)))
+        self.h = h.value
+        self.nlevels = nlevels
+        self.levels = []
+        for lv in range(nlevels):
+            info = (C.c_int64 * 8)()
+            _check(lib.hpcgx_level_info(self.h, lv, info))
+            self.levels.append(Level(info))
+        self.m = self.levels[0].m
+        ptrs = (C.c_void_p * 8)()
+        _check(lib.hpcgx_vectors(self.h, ptrs))
+        names = ("b", "x", "xexact", "r", "z", "p", "Ap", "z32")
+        self.vec = {}
+        for name, ptr in zip(names, ptrs):
+            if ptr:
+                dt = np.float32 if name == "z32" else np.float64
+                self.vec[name] = DeviceArray(self.m, dt, ptr=ptr, owner=False)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.hpcgx_problem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- introspection -------------------------------------------------------
+    def color_table(self, level=0):
+        L = self.levels[level]
+        sizes = np.zeros(L.ncolors, np.int32)
+        offsets = np.zeros(L.ncolors + 1, np.int32)
+        rounds = np.zeros(64, np.int32)
+        _check(lib.hpcgx_color_table(self.h, level, sizes.ctypes.data, offsets.ctypes.data,
+                                     rounds.ctypes.data))
+        return sizes, offsets, rounds
+
+    def level_arrays(self, level=0):
+        """Device arrays of a level copied to host numpy arrays."""
+        L = self.levels[level]
+        ptrs = (C.c_void_p * 10)()
+        _check(lib.hpcgx_level_arrays(self.h, level, ptrs))
+        m, mc = L.m, L.mc
+
+        def get(ptr, n, dt):
+            if not ptr or n == 0:
+                return None
+            return DeviceArray(n, dt, ptr=ptr, owner=False).host()
+        out = dict(
+            ell_col=get(ptrs[0], 27 * m, np.int32), ell_val=get(ptrs[1], 27 * m, np.float64),
+            ell_f16=get(ptrs[2], 27 * m, np.float16), diag_idx=get(ptrs[3], m, np.int32),
+            inv_diag=get(ptrs[4], m, np.float64), perm=get(ptrs[5], m, np.int32),
+            colors=get(ptrs[6], m, np.int32), f2c=get(ptrs[7], mc, np.int32),
+            f2c_fine=get(ptrs[8], mc, np.int32), f2c_coarse=get(ptrs[9], mc, np.int32))
+        return out
+
+    def make_rhs(self, kind=0, seed=0):
+        _check(lib.hpcgx_make_rhs(self.h, kind, seed))
+
+    # -- operators (reference names in the C ABI docs) -------------------------
+    def spmv(self, x, y, level=0):
+        _check(lib.hpcgx_spmv(self.h, level, _p(x), _p(y)))
+
+    def symgs(self, r, x, level=0):
+        _check(lib.hpcgx_symgs(self.h, level, _p(r), _p(x)))
+
+    def symgs_zero(self, r, x, level=0):
+        _check(lib.hpcgx_symgs_zero(self.h, level, _p(r), _p(x)))
+
+    def mg(self, r, x, level=0):
+        _check(lib.hpcgx_mg(self.h, level, _p(r), _p(x)))
+
+    def mg_mixed(self, r, x32):
+        _check(lib.hpcgx_mg_mixed(self.h, _p(r), _p(x32)))
+
+    def restriction(self, rf, xf, level=0):
+        out = C.c_void_p()
+        _check(lib.hpcgx_restriction(self.h, level, _p(rf), _p(xf), C.byref(out)))
+        return DeviceArray(self.levels[level + 1].m, np.float64, ptr=out.value, owner=False)
+
+    def prolongation(self, xf, level=0):
+        out = C.c_void_p()
+        _check(lib.hpcgx_prolongation(self.h, level, _p(xf), C.byref(out)))
+        return DeviceArray(self.levels[level + 1].m, np.float64, ptr=out.value, owner=False)
+
+    def dot(self, x, y, n=None):
+        res = C.c_double()
+        n = self.m if n is None else n
+        _check(lib.hpcgx_dot(self.h, n, _p(x), _p(y), C.byref(res)))
+        return res.value
+
+    def waxpby(self, alpha, x, beta, y, w, n=None):
+        n = self.m if n is None else n
+        _check(lib.hpcgx_waxpby(self.h, n, alpha, _p(x), beta, _p(y), _p(w)))
+
+    def fused_waxpby_dot(self, alpha, x, y, n=None):
+        res = C.c_double()
+        n = self.m if n is None else n
+        _check(lib.hpcgx_fused_waxpby_dot(self.h, n, alpha, _p(x), _p(y), C.byref(res)))
+        return res.value
+
+    def copy_diagonal(self, d, level=0):
+        _check(lib.hpcgx_copy_diagonal(self.h, level, _p(d)))
+
+    def replace_diagonal(self, d, level=0):
+        _check(lib.hpcgx_replace_diagonal(self.h, level, _p(d)))
+
+    def testcg_scale(self, d, b):
+        _check(lib.hpcgx_testcg_scale(self.h, _p(d), _p(b)))
+
+    def synchronize(self):
+        _check(lib.hpcgx_synchronize(self.h))
+
+    # -- CG ------------------------------------------------------------------
+    def cg(self, b=None, x=None, max_iter=50, tol=0.0, mode=CG_F64, precond=True, use_graph=True,
+           history=True):
+        """CG (CG.cpp:104-200); returns dict(niters, normr, normr0, history, seconds)."""
+        ni, nr, nr0 = C.c_int(), C.c_double(), C.c_double()
+        hist = np.zeros(max_iter + 1)
+        times = np.zeros(8)
+        _check(lib.hpcgx_cg(self.h, mode, _p(b), _p(x), max_iter, tol, int(precond), int(use_graph),
+                            C.byref(ni), C.byref(nr), C.byref(nr0),
+                            hist.ctypes.data if history else None, max_iter + 1, times.ctypes.data))
+        return dict(niters=ni.value, normr=nr.value, normr0=nr0.value,
+                    history=hist[:ni.value + 1].copy(), seconds=times[0])
+
+    def cg_begin(self, mode=CG_F64, b=None, x=None, max_iter=50, tol=0.0, precond=True):
+        _check(lib.hpcgx_cg_begin(self.h, mode, _p(b), _p(x), max_iter, tol, int(precond)))
+
+    def cg_iterate(self, count, use_graph=True):
+        _check(lib.hpcgx_cg_iterate(self.h, count, int(use_graph)))
+
+    def cg_iterate_profiled(self, count):
+        ms = np.zeros(len(PHASES))
+        _check(lib.hpcgx_cg_iterate_profiled(self.h, count, ms.ctypes.data))
+        return dict(zip(PHASES, ms))
+
+    def cg_end(self, hist_len=0):
+        ni, nr, nr0 = C.c_int(), C.c_double(), C.c_double()
+        hist = np.zeros(max(hist_len, 1))
+        _check(lib.hpcgx_cg_end(self.h, C.byref(ni), C.byref(nr), C.byref(nr0),
+                                hist.ctypes.data if hist_len else None, hist_len))
+        return dict(niters=ni.value, normr=nr.value, normr0=nr0.value,
+                    history=hist[:min(hist_len, ni.value + 1)])
+
+    def set_residual_replacement(self, every):
+        _check(lib.hpcgx_set_residual_replacement(self.h, every))
+
+    def flops_per_iteration(self):
+        return lib.hpcgx_flops_per_iteration(self.h)
+
+    def time_operator(self, op, level=0, reps=10):
+        ms = C.c_double()
+        _check(lib.hpcgx_time_operator(self.h, op, level, reps, C.byref(ms)))
+        return ms.value
+
+
+def version() -> str:
+    return lib.hpcgx_version().decode()

@@ -1,0 +1,448 @@
+// capi.cpp -- the extern "C" surface (include/rochpcg_amd.h) over the solver.
+#include "../../include/rochpcg_amd.h"
+
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "ops.hpp"
+#include "setup.hpp"
+#include "solver.hpp"
+
+namespace hpcgx
+{
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+}  // namespace hpcgx
+
+using namespace hpcgx;
+
+struct hpcgx_problem
+{
+    Problem* P;
+};
+
+#define CHECK_P(p)                                                                                                     \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if(!(p) || !(p)->P)                                                                                            \
+        {                                                                                                              \
+            set_error("null problem handle");                                                                          \
+            return -2;                                                                                                 \
+        }                                                                                                              \
+    } while(0)
+
+#define CHECK_LEVEL(p, l)                                                                                              \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if((l) < 0 || (l) >= (p)->P->nlevels)                                                                          \
+        {                                                                                                              \
+            set_error("level out of range");                                                                           \
+            return -2;                                                                                                 \
+        }                                                                                                              \
+    } while(0)
+
+#define NEED_F64(p)                                                                                                    \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if(!(p)->P->has_f64)                                                                                           \
+        {                                                                                                              \
+            set_error("operation needs the f64 store (HPCGX_STORE_F64)");                                              \
+            return -2;                                                                                                 \
+        }                                                                                                              \
+    } while(0)
+
+extern "C" {
+
+int hpcgx_problem_create(int nx, int ny, int nz, int nlevels, int stores, void* stream, hpcgx_problem** out)
+{
+    if(!out)
+    {
+        set_error("null output pointer");
+        return -2;
+    }
+    *out = nullptr;
+    Problem* P = nullptr;
+    int rc = problem_create(P, nx, ny, nz, nlevels, (stores & HPCGX_STORE_F64) != 0, (stores & HPCGX_STORE_MIXED) != 0,
+                            (hipStream_t)stream);
+    if(rc) return rc;
+    *out = new hpcgx_problem{P};
+    return 0;
+}
+
+int hpcgx_problem_destroy(hpcgx_problem* p)
+{
+    if(!p) return 0;
+    problem_destroy(p->P);
+    delete p;
+    return 0;
+}
+
+int hpcgx_level_info(const hpcgx_problem* p, int level, int64_t* info)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    const Level& L = p->P->L[level];
+    info[0] = L.nx;
+    info[1] = L.ny;
+    info[2] = L.nz;
+    info[3] = L.m;
+    info[4] = L.nnz;
+    info[5] = L.ncolors;
+    info[6] = L.mc;
+    info[7] = (L.ell_val ? HPCGX_STORE_F64 : 0) | (L.ell_h ? HPCGX_STORE_MIXED : 0);
+    return 0;
+}
+
+int hpcgx_color_table(const hpcgx_problem* p, int level, int* sizes, int* offsets, int* round_sizes)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    const Level& L = p->P->L[level];
+    for(int c = 0; c < L.ncolors; ++c)
+    {
+        if(sizes) sizes[c] = L.sizes[c];
+        if(offsets) offsets[c] = L.offsets[c];
+    }
+    if(offsets) offsets[L.ncolors] = L.offsets[L.ncolors];
+    if(round_sizes)
+    {
+        int nr = 0;
+        while(nr < MAX_COLORS && (nr < L.ncolors || L.round_sizes[nr] != 0)) ++nr;
+        for(int k = 0; k < nr; ++k) round_sizes[k] = L.round_sizes[k];
+    }
+    return 0;
+}
+
+int hpcgx_level_arrays(const hpcgx_problem* p, int level, void** ptrs)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    const Level& L = p->P->L[level];
+    void* v[] = {L.ell_col, L.ell_val, L.ell_h, L.diag_idx, L.inv_diag, L.perm, L.colors, L.f2c_nat, L.f2c_fine,
+                 L.f2c_coarse};
+    for(int i = 0; i < 10; ++i) ptrs[i] = v[i];
+    return 0;
+}
+
+int hpcgx_vectors(const hpcgx_problem* p, void** ptrs)
+{
+    CHECK_P(p);
+    Problem* P = p->P;
+    void* v[] = {P->b, P->x, P->xexact, P->r, P->z64, P->p, P->Ap, P->z32};
+    for(int i = 0; i < 8; ++i) ptrs[i] = v[i];
+    return 0;
+}
+
+int hpcgx_make_rhs(hpcgx_problem* p, int kind, uint64_t seed)
+{
+    CHECK_P(p);
+    if(kind != 0 && kind != 1)
+    {
+        set_error("rhs kind must be 0 or 1");
+        return -2;
+    }
+    HPCGX_TRY(make_rhs(p->P->L[0], kind, seed, p->P->b, p->P->xexact, p->P->x, p->P->s));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+int hpcgx_spmv(hpcgx_problem* p, int level, const double* x, double* y)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    const Level& L = p->P->L[level];
+    // f64 vectors over the f64 store, or over the exact f16 store when that is all there is
+    return launch_spmv_f64x(L, L.ell_val ? PREC_F64 : PREC_MIXED, x, y, nullptr, p->P->s, nullptr);
+}
+
+int hpcgx_symgs(hpcgx_problem* p, int level, const double* r, double* x)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    NEED_F64(p);
+    return launch_symgs(p->P->L[level], PREC_F64, r, true, x, p->P->v64[level].t, p->P->s, nullptr);
+}
+
+int hpcgx_symgs_zero(hpcgx_problem* p, int level, const double* r, double* x)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    NEED_F64(p);
+    return launch_symgs_zero(p->P->L[level], PREC_F64, r, true, x, p->P->s, nullptr);
+}
+
+int hpcgx_mg(hpcgx_problem* p, int level, const double* r, double* x)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    NEED_F64(p);
+    return mg(*p->P, level, PREC_F64, r, true, x, nullptr);
+}
+
+int hpcgx_mg_mixed(hpcgx_problem* p, const double* r, float* x)
+{
+    CHECK_P(p);
+    if(!p->P->has_mixed)
+    {
+        set_error("operation needs the mixed store (HPCGX_STORE_MIXED)");
+        return -2;
+    }
+    return mg(*p->P, 0, PREC_MIXED, r, true, x, nullptr);
+}
+
+int hpcgx_restriction(hpcgx_problem* p, int level, const double* rf, const double* xf, double** rc_out)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    NEED_F64(p);
+    if(level + 1 >= p->P->nlevels)
+    {
+        set_error("the coarsest level has no restriction");
+        return -2;
+    }
+    void* rc = p->P->v64[level + 1].r;
+    if(rc_out) *rc_out = (double*)rc;
+    return launch_restrict(p->P->L[level], PREC_F64, rf, true, xf, rc, p->P->s, nullptr);
+}
+
+int hpcgx_prolongation(hpcgx_problem* p, int level, double* xf, double** xc_out)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    NEED_F64(p);
+    if(level + 1 >= p->P->nlevels)
+    {
+        set_error("the coarsest level has no prolongation");
+        return -2;
+    }
+    void* xc = p->P->v64[level + 1].x;
+    if(xc_out) *xc_out = (double*)xc;
+    return launch_prolong(p->P->L[level], PREC_F64, xc, xf, p->P->s, nullptr);
+}
+
+static RedArgs store_red(Problem* P)
+{
+    RedArgs ra;
+    ra.partial = P->partial;
+    ra.ticket = P->ticket;
+    ra.op = OP_STORE;
+    ra.out = P->dout;
+    return ra;
+}
+
+int hpcgx_dot(hpcgx_problem* p, int64_t n, const double* x, const double* y, double* result)
+{
+    CHECK_P(p);
+    Problem* P = p->P;
+    HPCGX_TRY(launch_dot(n, x, true, y, true, store_red(P), P->s, nullptr));
+    HPCGX_HIP(hipMemcpyAsync(P->h_out, P->dout, sizeof(double), hipMemcpyDeviceToHost, P->s));
+    HPCGX_HIP(hipStreamSynchronize(P->s));
+    *result = P->h_out[0];
+    return 0;
+}
+
+int hpcgx_waxpby(hpcgx_problem* p, int64_t n, double alpha, const double* x, double beta, const double* y, double* w)
+{
+    CHECK_P(p);
+    return launch_waxpby(n, alpha, x, beta, y, w, p->P->s, nullptr);
+}
+
+int hpcgx_fused_waxpby_dot(hpcgx_problem* p, int64_t n, double alpha, const double* x, double* y, double* result)
+{
+    CHECK_P(p);
+    Problem* P = p->P;
+    HPCGX_TRY(launch_axpy_dot(n, alpha, x, y, store_red(P), P->s, nullptr));
+    HPCGX_HIP(hipMemcpyAsync(P->h_out, P->dout, sizeof(double), hipMemcpyDeviceToHost, P->s));
+    HPCGX_HIP(hipStreamSynchronize(P->s));
+    *result = P->h_out[0];
+    return 0;
+}
+
+int hpcgx_copy_diagonal(hpcgx_problem* p, int level, double* diag)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    return launch_get_diag(p->P->L[level], diag, p->P->s);
+}
+
+int hpcgx_replace_diagonal(hpcgx_problem* p, int level, const double* diag)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    return launch_set_diag(p->P->L[level], diag, p->P->s);
+}
+
+int hpcgx_testcg_scale(hpcgx_problem* p, double* diag, double* b)
+{
+    CHECK_P(p);
+    return launch_testcg_scale(p->P->L[0].m, diag, b, p->P->s);
+}
+
+// ---------------------------------------------------------------------------
+int hpcgx_cg(hpcgx_problem* p, int mode, const double* b, double* x, int max_iter, double tolerance, int precond,
+             int use_graph, int* niters, double* normr, double* normr0, double* history, int hist_len, double* times)
+{
+    CHECK_P(p);
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = cg_solve(*p->P, mode, precond != 0, b, x, max_iter, tolerance, use_graph != 0, niters, normr, normr0,
+                      history, hist_len);
+    if(times) times[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+int hpcgx_cg_begin(hpcgx_problem* p, int mode, const double* b, double* x, int max_iter, double tolerance, int precond)
+{
+    CHECK_P(p);
+    return cg_begin(*p->P, mode, precond != 0, b, x, max_iter, tolerance);
+}
+
+int hpcgx_cg_iterate(hpcgx_problem* p, int count, int use_graph)
+{
+    CHECK_P(p);
+    return cg_iterations(*p->P, count, use_graph != 0);
+}
+
+int hpcgx_cg_iterate_profiled(hpcgx_problem* p, int count, double* phase_ms)
+{
+    CHECK_P(p);
+    return cg_iterations_profiled(*p->P, count, phase_ms);
+}
+
+int hpcgx_cg_end(hpcgx_problem* p, int* niters, double* normr, double* normr0, double* history, int hist_len)
+{
+    CHECK_P(p);
+    return cg_end(*p->P, niters, normr, normr0, history, hist_len);
+}
+
+int hpcgx_set_residual_replacement(hpcgx_problem* p, int every)
+{
+    CHECK_P(p);
+    if(every < 0)
+    {
+        set_error("replacement interval must be >= 0");
+        return -2;
+    }
+    p->P->replace_every = every;
+    return 0;
+}
+
+double hpcgx_flops_per_iteration(const hpcgx_problem* p)
+{
+    if(!p || !p->P) return 0.0;
+    return hpcg_flops_per_iter(*p->P);
+}
+
+// ---------------------------------------------------------------------------
+int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int reps, double* avg_ms)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    Problem* P = p->P;
+    const Level& L = P->L[level];
+    const bool need64 = (op == 0 || op == 1 || op == 2 || op == 5);
+    const bool need32 = (op == 3 || op == 4 || op == 6);
+    if((need64 && !P->has_f64) || (need32 && !P->has_mixed) || op < 0 || op > 6 || reps < 1)
+    {
+        set_error("operator not available for this problem");
+        return -2;
+    }
+    if((op == 5 || op == 6) && level != 0)
+    {
+        set_error("MG timing starts at level 0");
+        return -2;
+    }
+    // operands: level-sized scratch that the CG owns (contents irrelevant)
+    double* a64 = (level == 0) ? P->p : (double*)P->v64[level].r;
+    double* b64 = (level == 0) ? P->Ap : (double*)P->v64[level].x;
+    if(!P->has_f64 && level > 0)
+    {
+        a64 = P->p;
+        b64 = P->Ap;
+    }
+    auto run = [&]() -> int {
+        switch(op)
+        {
+        case 0: return launch_spmv_f64x(L, PREC_F64, a64, b64, nullptr, P->s, nullptr);
+        case 1: return launch_symgs(L, PREC_F64, a64, true, b64, P->v64[level].t, P->s, nullptr);
+        case 2: return launch_symgs_zero(L, PREC_F64, a64, true, b64, P->s, nullptr);
+        case 3: return launch_spmv_f64x(L, PREC_MIXED, P->p, P->Ap, nullptr, P->s, nullptr);
+        case 4:
+            return launch_symgs(L, PREC_MIXED, level == 0 ? (void*)P->r : P->v32[level].r, level == 0,
+                                level == 0 ? (void*)P->z32 : P->v32[level].x, P->v32[level].t, P->s, nullptr);
+        case 5: return mg(*P, 0, PREC_F64, P->r, true, P->z64, nullptr);
+        case 6: return mg(*P, 0, PREC_MIXED, P->r, true, P->z32, nullptr);
+        }
+        return -2;
+    };
+    HPCGX_TRY(run());   // warm-up
+    hipEvent_t e0, e1;
+    HPCGX_HIP(hipEventCreate(&e0));
+    HPCGX_HIP(hipEventCreate(&e1));
+    HPCGX_HIP(hipEventRecord(e0, P->s));
+    for(int i = 0; i < reps; ++i) HPCGX_TRY(run());
+    HPCGX_HIP(hipEventRecord(e1, P->s));
+    HPCGX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HPCGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    return 0;
+}
+
+int hpcgx_synchronize(hpcgx_problem* p)
+{
+    CHECK_P(p);
+    HPCGX_HIP(hipStreamSynchronize(p->P->s));
+    return 0;
+}
+
+int hpcgx_malloc(void** ptr, size_t bytes)
+{
+    HPCGX_HIP(hipMalloc(ptr, bytes ? bytes : 8));
+    return 0;
+}
+
+int hpcgx_free(void* ptr)
+{
+    if(ptr) HPCGX_HIP(hipFree(ptr));
+    return 0;
+}
+
+int hpcgx_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    // solver streams are non-blocking: finish their queued work first
+    HPCGX_HIP(hipDeviceSynchronize());
+    HPCGX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int hpcgx_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    HPCGX_HIP(hipDeviceSynchronize());
+    HPCGX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int hpcgx_memcpy_d2d(void* dst, const void* src, size_t bytes)
+{
+    HPCGX_HIP(hipDeviceSynchronize());
+    HPCGX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    HPCGX_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+int hpcgx_device_count(int* count)
+{
+    *count = 0;
+    HPCGX_HIP(hipGetDeviceCount(count));
+    return 0;
+}
+
+const char* hpcgx_last_error(void) { return last_error(); }
+
+const char* hpcgx_version(void) { return "rochpcg_amd 0.1 (gfx950)"; }
+
+}  // extern "C"

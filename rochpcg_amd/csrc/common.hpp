@@ -1,0 +1,174 @@
+// common.hpp -- shared types and device helpers of the MI355X (gfx950) HPCG
+// solver library.  Everything here is internal; the public surfaces are
+// include/rochpcg_amd.h (C ABI) and include/hpcg/*.hpp (reference-named C++).
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace hpcgx
+{
+
+constexpr int W = 27;            // ELL width of the 27-point operator
+constexpr int MAX_COLORS = 32;   // colours a level may hold
+constexpr int MAX_LEVELS = 8;
+constexpr int NUM_XCD = 8;       // XCDs of an MI355X
+
+// Error reporting: every entry point returns an int status; the last message
+// is kept per thread for hpcgx_last_error().
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define HPCGX_HIP(expr)                                                                       \
+    do                                                                                         \
+    {                                                                                          \
+        hipError_t e_ = (expr);                                                                \
+        if(e_ != hipSuccess)                                                                   \
+        {                                                                                      \
+            ::hpcgx::set_error(std::string(#expr) + ": " + hipGetErrorString(e_) + " at " +   \
+                               __FILE__ + ":" + std::to_string(__LINE__));                     \
+            return -1;                                                                         \
+        }                                                                                      \
+    } while(0)
+
+#define HPCGX_TRY(expr)                                                                        \
+    do                                                                                         \
+    {                                                                                          \
+        int s_ = (expr);                                                                       \
+        if(s_ != 0) return s_;                                                                 \
+    } while(0)
+
+// Precision of a level's arrays.
+enum Prec : int
+{
+    PREC_F64 = 0,   // f64 ELL values, f64 vectors
+    PREC_MIXED = 1  // f16 ELL values, f32 vectors
+};
+
+// Launch shape of one operator on one level (chosen by the autotuner).
+struct Shape
+{
+    int block = 256;   // threads per workgroup (multiple of 64)
+};
+
+// One level of the hierarchy, device pointers plus host-side colour table.
+// Rows are in multicolour order: colour c owns rows [offsets[c], offsets[c+1]).
+struct Level
+{
+    int nx = 0, ny = 0, nz = 0;
+    int64_t m = 0;          // rows
+    int64_t nnz = 0;        // stored nonzeros
+    int ncolors = 0;
+    int sizes[MAX_COLORS] = {};
+    int offsets[MAX_COLORS + 1] = {};
+    int round_sizes[MAX_COLORS] = {};   // per JPL round, as the reference records them
+
+    // column-major ELL: entry p of row r lives at p*m + r (64-bit offsets)
+    int32_t* ell_col = nullptr;   // [W*m], -1 padding at the tail of each row
+    double* ell_val = nullptr;    // [W*m] (f64 levels)
+    __half* ell_h = nullptr;      // [W*m] (mixed levels)
+    int32_t* diag_idx = nullptr;  // [m] slot of the diagonal
+    double* inv_diag = nullptr;   // [m] 1/a_ii (f64)
+    float* diag_f = nullptr;      // [m] a_ii in f32 (mixed levels)
+    int32_t* perm = nullptr;      // [m] natural row -> multicolour row
+    int32_t* colors = nullptr;    // [m] colour of each natural row
+    int32_t* iperm_tmp = nullptr; // [m] multicolour row -> natural row (setup only)
+
+    // transfer to the next coarser level (indexed by coarse natural row k)
+    int64_t mc = 0;
+    int32_t* f2c_fine = nullptr;    // [mc] multicolour fine row injected into k
+    int32_t* f2c_coarse = nullptr;  // [mc] multicolour coarse row of k
+    int32_t* f2c_nat = nullptr;     // [mc] natural fine row (reference f2cOperator)
+
+    Shape spmv, sweep, restrict_, prolong;
+};
+
+// Device-side CG scalars (one cache line each to avoid false sharing is not
+// needed: a single block writes them).
+struct CGScalars
+{
+    double rtz;
+    double oldrtz;
+    double pAp;
+    double alpha;
+    double beta;
+    double normr;
+    double normr0;
+    double tol;
+    int iter;       // iterations completed
+    int done;       // 1 once normr/normr0 <= tol or max_iter reached
+    int max_iter;
+    int pad;
+};
+
+// Scratch for deterministic two-pass reductions: per-block partials plus a
+// ticket counter; the last block to finish folds the partials in index order.
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+struct Reduce
+{
+    double* partial = nullptr;      // [RED_BLOCKS * 2]
+    unsigned int* ticket = nullptr; // [4]
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+template <typename T>
+__device__ __forceinline__ T ntload(const T* p)
+{
+    return __builtin_nontemporal_load(p);
+}
+__device__ __forceinline__ __half ntload(const __half* p)
+{
+    return __ushort_as_half(__builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p)));
+}
+
+// Contiguous block ranges per XCD: the hardware hands workgroup i to XCD
+// i % 8, so logical block L = base(i % 8) + i / 8 gives each XCD one
+// contiguous slice of the row space (its L2 then serves the x-gathers of
+// neighbouring rows).
+__device__ __forceinline__ int64_t xcd_block(int64_t i, int64_t nblocks)
+{
+    const int64_t q = nblocks / NUM_XCD;
+    const int64_t rr = nblocks % NUM_XCD;
+    const int64_t x = i % NUM_XCD;
+    const int64_t base = x * q + (x < rr ? x : rr);
+    return base + i / NUM_XCD;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sum of BS threads (BS/64 waves), result valid in thread 0.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* lds)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    if(lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if(threadIdx.x == 0)
+    {
+#pragma unroll
+        for(int w = 0; w < BS / 64; ++w) s += lds[w];
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace hpcgx

@@ -1,0 +1,56 @@
+// ops.hpp -- host launchers of the hot-path kernels (ops.hip).
+#pragma once
+#include "common.hpp"
+
+namespace hpcgx
+{
+
+// What the last block of a reduction does with the folded sum.
+enum RedOp : int
+{
+    OP_NONE = 0,
+    OP_STORE = 1,   // *out = sum
+    OP_NORMR0 = 2,  // normr = normr0 = sqrt(sum); iteration 0 of the history
+    OP_RTZ = 3,     // oldrtz = rtz; rtz = sum; beta = rtz/oldrtz (after the first iteration)
+    OP_PAP = 4,     // pAp = sum; alpha = rtz/pAp
+    OP_NORMR = 5,   // normr = sqrt(sum); iter += 1; history; convergence flag
+    OP_NORMR_SET = 6 // normr = sqrt(sum) of a recomputed true residual (same iteration)
+};
+
+struct RedArgs
+{
+    double* partial = nullptr;      // >= RED_BLOCKS doubles
+    unsigned int* ticket = nullptr; // zero between reductions
+    int op = OP_NONE;
+    CGScalars* sc = nullptr;
+    double* hist = nullptr;
+    int hist_cap = -1;
+    double* out = nullptr;
+};
+
+// `gate` (device int, may be null): kernels return at once while *gate != 0.
+int launch_spmv(const Level& L, Prec prec, const void* x, void* y, hipStream_t s, const int* gate);
+int launch_spmv_f64x(const Level& L, Prec prec, const double* x, double* y, const RedArgs* ra, hipStream_t s,
+                     const int* gate);
+int launch_symgs_zero(const Level& L, Prec prec, const void* r, bool r_f64, void* x, hipStream_t s, const int* gate);
+int launch_symgs(const Level& L, Prec prec, const void* r, bool r_f64, void* x, void* t, hipStream_t s, const int* gate);
+int launch_restrict(const Level& F, Prec prec, const void* r, bool r_f64, const void* x, void* rc, hipStream_t s,
+                    const int* gate);
+int launch_prolong(const Level& F, Prec prec, const void* xc, void* x, hipStream_t s, const int* gate);
+int launch_dot(int64_t n, const void* x, bool x_f64, const void* y, bool y_f64, const RedArgs& ra, hipStream_t s,
+               const int* gate);
+int launch_waxpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* w, hipStream_t s,
+                  const int* gate);
+int launch_axpy_dot(int64_t n, double alpha, const double* x, double* y, const RedArgs& ra, hipStream_t s,
+                    const int* gate);
+int launch_cg_p(int64_t n, const void* z, bool z_f64, double* p, const CGScalars* sc, hipStream_t s,
+                const int* gate);
+int launch_cg_xr(int64_t n, const double* p, const double* Ap, double* x, double* r, const RedArgs& ra,
+                 hipStream_t s, const int* gate);
+int launch_fill(int64_t n, void* p, bool f64, double v, hipStream_t s, const int* gate);
+int launch_copy(int64_t n, const void* src, bool src_f64, void* dst, bool dst_f64, hipStream_t s, const int* gate);
+int launch_get_diag(const Level& L, double* d, hipStream_t s);
+int launch_set_diag(Level& L, const double* d, hipStream_t s);
+int launch_testcg_scale(int64_t m, double* d, double* b, hipStream_t s);
+
+}  // namespace hpcgx

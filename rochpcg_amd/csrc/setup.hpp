@@ -1,0 +1,17 @@
+// setup.hpp -- device-side construction of the multicolour hierarchy.
+#pragma once
+#include "common.hpp"
+
+namespace hpcgx
+{
+// JPL colouring + colour-stable permutation of one level (fills colors, perm,
+// ncolors/sizes/offsets/round_sizes, and a temporary inverse permutation).
+int color_level(Level& L, hipStream_t s);
+// Colouring plus the multicolour column-major ELL operator of an nx*ny*nz grid.
+int build_level(Level& L, int nx, int ny, int nz, bool with_f64, bool with_f16, hipStream_t s);
+// Injection maps from fine level F to the next coarser level C.
+int build_transfer(Level& F, Level& C, hipStream_t s);
+// b / xexact / x0 in multicolour order (kind 0: xexact = 1; kind 1: seeded uniform [-1,1]).
+int make_rhs(const Level& L, int kind, uint64_t seed, double* b, double* xexact, double* x0, hipStream_t s);
+void free_level(Level& L);
+}  // namespace hpcgx

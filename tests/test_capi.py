@@ -1,0 +1,40 @@
+"""The C ABI library loads without a GPU and exports every declared entry point."""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hpcgx_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import rochpcg_amd
+    lib = ctypes.CDLL(rochpcg_amd.LIB_PATH)
+    names = declared("rochpcg_amd.h")
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_errors_are_reported_not_fatal():
+    import pytest
+    import rochpcg_amd
+    # invalid sizes are rejected by the host checks before any device call
+    with pytest.raises(rochpcg_amd.HPCGError, match="divisible"):
+        rochpcg_amd.Problem(30, 30, 30, nlevels=4)
+    with pytest.raises(rochpcg_amd.HPCGError, match="invalid"):
+        rochpcg_amd.Problem(1, 8, 8, nlevels=1)
+    assert "gfx950" in rochpcg_amd.version()
+
+
+def test_hpcg_flop_model():
+    from rochpcg_amd import vv
+    # ReportResults.cpp:73-89 at 320^3 (12.177 GFLOP per iteration)
+    f = vv.hpcg_flops_per_iteration((320, 320, 320))
+    assert abs(f / 1e9 - 12.177) < 0.001
+    assert vv.check_niter(32) == 53 and vv.check_niter(288) == 51 and vv.check_niter(320) == 50

@@ -1,0 +1,228 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle.
+
+Integer/index work (colours, perm, ELL columns and values, diagonal slots,
+injection maps, right-hand sides) must match bit for bit.  Floating-point
+operators are compared normwise with the tolerances the contract states:
+SpMV 1e-14 relative, CG residual history 1e-12 relative per iteration (FP64),
+SymGS/MG 1e-12 relative (same colour order; the GPU uses fma and skips the
+last colour of the backward sweep, which is the same arithmetic up to rounding).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 16, 16), (32, 32, 32), (24, 16, 32), (32, 24, 16)]
+
+
+def _hip_problem(hip, dims, nlevels=4, stores=None):
+    print(f"[gpu] problem {dims} levels={nlevels}", flush=True)
+    stores = hip.STORE_F64 | hip.STORE_MIXED if stores is None else stores
+    return hip.Problem(*dims, nlevels=nlevels, stores=stores)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("dims", SHAPES)
+def test_setup_bit_exact(hip, oracle_mod, dims):
+    G = _hip_problem(hip, dims)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    for lv in range(4):
+        g = G.level_arrays(lv)
+        o = O.export(lv)
+        L = G.levels[lv]
+        m = L.m
+        assert (L.nx, L.ny, L.nz, L.m) == (O.levels[lv].nx, O.levels[lv].ny, O.levels[lv].nz, O.levels[lv].m)
+        assert L.nnz == O.levels[lv].nnz_total
+        assert L.ncolors == O.levels[lv].ncolors
+        assert np.array_equal(g["colors"], o["colors"]), f"colors level {lv}"
+        assert np.array_equal(g["perm"], o["perm"]), f"perm level {lv}"
+        sizes, offsets, _ = G.color_table(lv)
+        assert np.array_equal(sizes, o["sizes"]) and np.array_equal(offsets, o["offsets"])
+        cols = g["ell_col"].reshape(27, m).T
+        assert np.array_equal(cols, o["cols"]), f"ell_col level {lv}"
+        assert np.array_equal(g["ell_val"].reshape(27, m).T, o["vals"]), f"ell_val level {lv}"
+        assert np.array_equal(g["ell_f16"].reshape(27, m).T.astype(np.float64), o["vals"])
+        assert np.array_equal(g["diag_idx"], o["diag"])
+        assert np.all(g["inv_diag"] == 1.0 / 26.0)
+        if lv + 1 < 4:
+            # oracle f2c is indexed by the coarse multicolour row
+            assert np.array_equal(o["f2c"][g["f2c_coarse"]], g["f2c_fine"])
+            nf = oracle_mod.f2c(L.nx, L.ny, L.nz)
+            assert np.array_equal(g["f2c"], nf)
+    b = G.vec["b"].host()
+    assert np.array_equal(b, O.rhs_ones())
+    assert np.all(G.vec["xexact"].host() == 1.0)
+    assert np.all(G.vec["x"].host() == 0.0)
+    G.make_rhs(1, 1234)
+    xs, bo = O.rhs_random(1234)
+    assert np.array_equal(G.vec["xexact"].host(), xs)
+    assert np.array_equal(G.vec["b"].host(), bo)
+    G.close()
+
+
+def test_setup_equal_hash_level(hip, oracle_mod):
+    """32x24x16 has a 4x3x2 coarsest level, where the reference's JPL never
+    terminates; the tie-broken colouring must match the oracle's."""
+    dims = (32, 24, 16)
+    G = _hip_problem(hip, dims)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    g, o = G.level_arrays(3), O.export(3)
+    assert np.array_equal(g["colors"], o["colors"])
+    assert np.array_equal(g["ell_col"].reshape(27, -1).T, o["cols"])
+
+
+@pytest.mark.parametrize("dims", SHAPES[:3])
+def test_operators_match_oracle(hip, oracle_mod, dims):
+    G = _hip_problem(hip, dims)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    rng = np.random.default_rng(7)
+    for lv in range(4):
+        m = G.levels[lv].m
+        x = rng.uniform(-1, 1, m)
+        dx, dy = hip.DeviceArray.from_host(x), hip.DeviceArray(m)
+        G.spmv(dx, dy, level=lv)
+        assert _rel(dy.host(), O.spmv(x, lv)) <= 1e-14, f"spmv level {lv}"
+        r = rng.uniform(-1, 1, m)
+        x0 = rng.uniform(-1, 1, m)
+        dr, dxx = hip.DeviceArray.from_host(r), hip.DeviceArray.from_host(x0)
+        G.symgs(dr, dxx, level=lv)
+        assert _rel(dxx.host(), O.symgs(r, x0.copy(), lv)) <= 1e-12, f"symgs level {lv}"
+        dz = hip.DeviceArray.from_host(np.full(m, 3.0))
+        G.symgs_zero(dr, dz, level=lv)
+        assert _rel(dz.host(), O.symgs(r, np.zeros(m), lv)) <= 1e-12, f"symgs0 level {lv}"
+    r = rng.uniform(-1, 1, G.m)
+    dr, dz = hip.DeviceArray.from_host(r), hip.DeviceArray(G.m)
+    G.mg(dr, dz)
+    assert _rel(dz.host(), O.mg(r)) <= 1e-12
+    # fused residual+restriction and prolongation on level 0
+    xf = rng.uniform(-1, 1, G.m)
+    dxf = hip.DeviceArray.from_host(xf)
+    rc = G.restriction(dr, dxf, level=0).host()
+    o = O.export(0)
+    res = r - O.spmv(xf)
+    rc_o = res[o["f2c"]]
+    assert _rel(rc, rc_o) <= 1e-13
+    xc = G.prolongation(dxf, level=0)
+    xcv = rng.uniform(-1, 1, G.levels[1].m)
+    xc.upload(xcv)
+    dxf.upload(xf)
+    G.prolongation(dxf, level=0)
+    expect = xf.copy()
+    expect[o["f2c"]] += xcv
+    assert np.array_equal(dxf.host(), expect)
+    G.close()
+
+
+def test_vector_ops(hip):
+    G = _hip_problem(hip, (32, 32, 32), stores=hip.STORE_F64)
+    rng = np.random.default_rng(3)
+    n = G.m
+    a, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    da, db, dw = hip.DeviceArray.from_host(a), hip.DeviceArray.from_host(b), hip.DeviceArray(n)
+    assert abs(G.dot(da, db) - a @ b) <= 1e-13 * np.abs(a) @ np.abs(b)
+    assert abs(G.dot(da, da) - a @ a) <= 1e-13 * (a @ a)
+    for alpha, beta in ((1.0, -0.5), (0.25, 1.0), (0.3, 0.7)):
+        G.waxpby(alpha, da, beta, db, dw)
+        assert np.allclose(dw.host(), alpha * a + beta * b, rtol=1e-15, atol=1e-15)
+    res = G.fused_waxpby_dot(-0.5, da, db)
+    yb = b - 0.5 * a
+    assert np.allclose(db.host(), yb, rtol=1e-15, atol=1e-15)
+    assert abs(res - yb @ yb) <= 1e-13 * (yb @ yb)
+    # a length that is not a multiple of any block size
+    k = n - 123
+    assert abs(G.dot(da, db, n=k) - a[:k] @ db.host()[:k]) <= 1e-12 * abs(a[:k]) @ abs(yb[:k])
+    G.close()
+
+
+@pytest.mark.parametrize("dims", [(32, 32, 32), (24, 16, 32)])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_cg_history_fp64(hip, oracle_mod, dims, use_graph):
+    """Residual history of 50 FP64 iterations vs CG_ref in the same colour order."""
+    G = _hip_problem(hip, dims, stores=hip.STORE_F64)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    res = G.cg(max_iter=50, tol=0.0, use_graph=use_graph)
+    ref = O.cg(O.rhs_ones(), max_iter=50)
+    assert res["niters"] == 50
+    h, hr = res["history"], ref["history"]
+    assert h[0] == hr[0] or abs(h[0] - hr[0]) <= 1e-14 * hr[0]
+    rel = np.abs(h / h[0] - hr / hr[0]) / (hr / hr[0])
+    print(f"[gpu] max per-iteration relative history difference {rel.max():.3e}", flush=True)
+    assert rel.max() <= 1e-12
+    G.close()
+
+
+def test_cg_graph_equals_eager(hip):
+    G = _hip_problem(hip, (32, 32, 32), stores=hip.STORE_F64)
+    a = G.cg(max_iter=30, use_graph=False)
+    x_eager = G.vec["x"].host()
+    G.vec["x"].upload(np.zeros(G.m))
+    b = G.cg(max_iter=30, use_graph=True)
+    assert np.array_equal(a["history"], b["history"])
+    assert np.array_equal(x_eager, G.vec["x"].host())
+    # replays after the first keep agreeing
+    G.vec["x"].upload(np.zeros(G.m))
+    c = G.cg(max_iter=30, use_graph=True)
+    assert np.array_equal(a["history"], c["history"])
+    G.close()
+
+
+def test_cg_convergence_and_reference_checks(hip, oracle_mod):
+    """refTolerance from 50 natural-order CG_ref iterations; the GPU CG reaches
+    it within check_niter (test_rochpcg.cpp:88-119, 323-324)."""
+    from rochpcg_amd import vv
+    dims = (32, 32, 32)
+    N = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_NATURAL)
+    ref = N.cg(N.rhs_ones(), max_iter=50)
+    ref_tol = ref["normr"] / ref["normr0"]
+    G = _hip_problem(hip, dims, stores=hip.STORE_F64)
+    res = G.cg(max_iter=500, tol=ref_tol)
+    assert res["normr"] / res["normr0"] <= ref_tol
+    assert res["niters"] <= vv.check_niter(32)
+    # the solution: x* = ones
+    x = G.vec["x"].host()
+    assert np.max(np.abs(x - 1.0)) < 1e-6
+    G.close()
+
+
+def test_testsymmetry_testcg_testnorms(hip):
+    from rochpcg_amd import vv
+    G = _hip_problem(hip, (32, 32, 32), stores=hip.STORE_F64)
+    sym = vv.testsymmetry(G)
+    assert sym["count_fail"] == 0, sym
+    tcg = vv.testcg(G)
+    assert tcg["count_fail"] == 0, tcg
+    assert tcg["niters_max_no_prec"] <= 12 and tcg["niters_max_prec"] <= 2
+    norms = []
+    for _ in range(3):
+        G.vec["x"].upload(np.zeros(G.m))
+        r = G.cg(max_iter=50, tol=0.0)
+        norms.append(r["normr"] / r["normr0"])
+    assert vv.testnorms(norms)["pass"]
+    assert len(set(norms)) == 1   # the device reductions are deterministic
+    G.close()
+
+
+@pytest.mark.parametrize("dims", [(32, 32, 32), (32, 24, 16)])
+def test_mixed_precision_reaches_1e9(hip, oracle_mod, dims):
+    G = _hip_problem(hip, dims)
+    G.make_rhs(1, 1234)
+    f64 = G.cg(max_iter=200, tol=1e-9, mode=hip.CG_F64)
+    x64 = G.vec["x"].host()
+    G.vec["x"].upload(np.zeros(G.m))
+    mix = G.cg(max_iter=200, tol=1e-9, mode=hip.CG_MIXED)
+    xm = G.vec["x"].host()
+    xs = G.vec["xexact"].host()
+    # true residual of the mixed solution in f64 (oracle SpMV, same order)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    _, b = O.rhs_random(1234)
+    rtrue = np.linalg.norm(b - O.spmv(xm)) / np.linalg.norm(b)
+    print(f"[gpu] {dims}: f64 {f64['niters']} it, mixed {mix['niters']} it, "
+          f"true rel residual {rtrue:.2e}, err {np.linalg.norm(xm - xs) / np.linalg.norm(xs):.2e}", flush=True)
+    assert mix["normr"] / mix["normr0"] <= 1e-9
+    assert rtrue <= 2e-9
+    assert mix["niters"] <= f64["niters"] + 3
+    assert np.linalg.norm(xm - x64) / np.linalg.norm(x64) < 1e-7
+    G.close()
