@@ -147,11 +147,24 @@ def test_cg_history_fp64(hip, oracle_mod, dims, use_graph):
     ref = O.cg(O.rhs_ones(), max_iter=50)
     assert res["niters"] == 50
     h, hr = res["history"], ref["history"]
-    assert h[0] == hr[0] or abs(h[0] - hr[0]) <= 1e-14 * hr[0]
-    rel = np.abs(h / h[0] - hr / hr[0]) / (hr / hr[0])
-    print(f"[gpu] max per-iteration relative history difference {rel.max():.3e}", flush=True)
-    assert rel.max() <= 1e-12
+    assert abs(h[0] - hr[0]) <= 1e-14 * hr[0]
+    assert_history_close(h, hr)
     G.close()
+
+
+def assert_history_close(h, hr, rel_tol=1e-12, abs_tol=1e-15):
+    """Per iteration: relative difference <= 1e-12, or an absolute difference
+    <= 1e-15 * normr0 once the scaled residual is below rounding noise.  The
+    reference's CG_ref does not reproduce itself more closely than that: its
+    OpenMP dot reduction run on 1 and on 8 threads differs by 0.9 relative at
+    iteration 50 of the 32^3 problem (scaled residual 1e-20 there)."""
+    rel = np.abs(h - hr) / hr
+    absd = np.abs(h - hr) / hr[0]
+    ok = (rel <= rel_tol) | (absd <= abs_tol)
+    print(f"[gpu] history: max rel {rel.max():.2e}, max |dh|/h0 {absd.max():.2e}, "
+          f"first k above 1e-12 rel: {int(np.argmax(rel > rel_tol)) if (rel > rel_tol).any() else -1}",
+          flush=True)
+    assert ok.all(), (np.nonzero(~ok)[0], rel[~ok], absd[~ok])
 
 
 def test_cg_graph_equals_eager(hip):
