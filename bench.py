@@ -99,6 +99,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the solve-to-1e-9 comparison")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of a hipGraph")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (256 threads, 1 row)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs (--gpus > 1) "
@@ -115,6 +116,16 @@ def main():
     log(f"setup {dims} on the GPU: {time.perf_counter() - t0:.2f} s")
     flops = G.flops_per_iteration()
     K, W = args.steps, args.warmup
+    tuned = {}
+    if not args.no_tune:
+        from rochpcg_amd import autotune
+        t0 = time.perf_counter()
+        table = autotune.tune(G, mixed=False, reps=3, log=log)
+        table = autotune.tune(G, mixed=True, reps=3, table=table, log=log)
+        log(f"autotune: {time.perf_counter() - t0:.1f} s")
+        L0 = G.levels[0]
+        tuned = {k: {op: v["shape"] for op, v in e.items()} for k, e in table.items()
+                 if k.startswith(f"{L0.nx}x{L0.ny}x{L0.nz}:")}
     use_graph = not args.eager
 
     def timed(mode):
@@ -166,7 +177,7 @@ def main():
         "config": {"workload": f"HPCG 27-point {dims[0]}x{dims[1]}x{dims[2]}, 4-level MG "
                                f"(1 pre/1 post multicolour SymGS), FP64 CG, b=A*ones, x0=0",
                    "nx": dims[0], "ny": dims[1], "nz": dims[2], "levels": 4,
-                   "graph": use_graph, "parallelism": "single GPU"},
+                   "graph": use_graph, "parallelism": "single GPU", "level0_shapes": tuned},
         "hbm_TBps": round(b64 / (ms64 * 1e-3) / 1e12, 3),
         "bytes_per_iteration": int(b64),
         "mixed": {"value": round(flops / (msmx * 1e-3) / 1e9, 2), "unit": "GFLOP/s",

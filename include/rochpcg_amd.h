@@ -143,12 +143,24 @@ double hpcgx_flops_per_iteration(const hpcgx_problem* p);
 
 /* ---- timing and memory helpers ----------------------------------------- */
 
-/* average device time (ms) of `reps` back-to-back launches of one operator
- * on `level`, measured with HIP events on the problem's stream:
- * op 0 = SpMV (f64), 1 = SymGS (f64), 2 = zero-guess SymGS (f64),
- * 3 = SpMV (f16 values, f64 vectors), 4 = SymGS (mixed), 5 = MG (f64),
- * 6 = MG (mixed) */
-int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int reps, double* avg_ms);
+/* average device time (ms) of `reps` back-to-back runs of one operator on
+ * `level`, measured with HIP events on the problem's stream, with the
+ * operands the V-cycle uses there (level 0: the CG residual r and z).
+ * op 0 = SpMV, 1 = SymGS, 2 = zero-guess SymGS, 3 = MG from `level`,
+ * 4/5 = zero-guess forward/backward sweeps, 6/7 = forward/backward sweeps,
+ * 8 = fused residual + restriction; prec 0 = f64 store, 1 = mixed store
+ * (SpMV: f16 values with f64 vectors). */
+int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps, double* avg_ms);
+/* launch shape of an operator on a level (the autotuner's knobs; no
+ * reference counterpart -- the reference hard-codes 1024/256/128-thread
+ * blocks, ComputeSPMV.cpp:262, ComputeSYMGS.cpp:395): op 0 = SpMV,
+ * 1 = zero-guess forward sweep, 2 = zero-guess backward sweep, 3 = forward
+ * sweep, 4 = backward sweep, 5 = restriction; block in {64,128,256} threads,
+ * rows in {1,2,4} consecutive rows per thread (restriction: 1).  level -1 /
+ * op -1 apply to all; prec 0 = f64 kernels, 1 = mixed kernels.  Colour
+ * blocks not divisible by `rows` run with 1. */
+int hpcgx_set_shape(hpcgx_problem* p, int prec, int level, int op, int block, int rows);
+int hpcgx_get_shape(const hpcgx_problem* p, int prec, int level, int op, int* block, int* rows);
 int hpcgx_synchronize(hpcgx_problem* p);
 int hpcgx_malloc(void** ptr, size_t bytes);
 int hpcgx_free(void* ptr);

@@ -25,8 +25,11 @@ CG_F64 = 0
 CG_MIXED = 1
 
 # hpcgx_time_operator op codes
-OP_SPMV, OP_SYMGS, OP_SYMGS_ZERO, OP_SPMV_F16, OP_SYMGS_MIXED, OP_MG, OP_MG_MIXED = range(7)
+(OP_SPMV, OP_SYMGS, OP_SYMGS_ZERO, OP_MG, OP_FWD_ZERO, OP_BWD_ZERO, OP_FWD_FULL, OP_BWD_FULL,
+ OP_RESTRICT) = range(9)
 PHASES = ("spmv", "mg", "ddot", "waxpby", "true_residual")
+# launch-shape operators (hpcgx_set_shape)
+SH_SPMV, SH_FWD_ZERO, SH_BWD_ZERO, SH_FWD_FULL, SH_BWD_FULL, SH_RESTRICT = range(6)
 
 
 class HPCGError(RuntimeError):
@@ -69,7 +72,9 @@ def _load():
         "hpcgx_cg_end": ([vp, P(i32), P(f64), P(f64), vp, i32], i32),
         "hpcgx_set_residual_replacement": ([vp, i32], i32),
         "hpcgx_flops_per_iteration": ([vp], f64),
-        "hpcgx_time_operator": ([vp, i32, i32, i32, P(f64)], i32),
+        "hpcgx_time_operator": ([vp, i32, i32, i32, i32, P(f64)], i32),
+        "hpcgx_set_shape": ([vp, i32, i32, i32, i32, i32], i32),
+        "hpcgx_get_shape": ([vp, i32, i32, i32, P(i32), P(i32)], i32),
         "hpcgx_synchronize": ([vp], i32),
         "hpcgx_malloc": ([P(vp), C.c_size_t], i32),
         "hpcgx_free": ([vp], i32),
@@ -289,6 +294,14 @@ class Problem:
     def testcg_scale(self, d, b):
         _check(lib.hpcgx_testcg_scale(self.h, _p(d), _p(b)))
 
+    def set_shape(self, block, rows, level=-1, op=-1, mixed=False):
+        _check(lib.hpcgx_set_shape(self.h, int(mixed), level, op, block, rows))
+
+    def get_shape(self, level, op, mixed=False):
+        b, r = C.c_int(), C.c_int()
+        _check(lib.hpcgx_get_shape(self.h, int(mixed), level, op, C.byref(b), C.byref(r)))
+        return b.value, r.value
+
     def synchronize(self):
         _check(lib.hpcgx_synchronize(self.h))
 
@@ -330,9 +343,9 @@ class Problem:
     def flops_per_iteration(self):
         return lib.hpcgx_flops_per_iteration(self.h)
 
-    def time_operator(self, op, level=0, reps=10):
+    def time_operator(self, op, level=0, mixed=False, reps=10):
         ms = C.c_double()
-        _check(lib.hpcgx_time_operator(self.h, op, level, reps, C.byref(ms)))
+        _check(lib.hpcgx_time_operator(self.h, op, level, int(mixed), reps, C.byref(ms)))
         return ms.value
 
 

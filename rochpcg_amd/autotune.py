@@ -1,0 +1,94 @@
+"""Per-level launch-shape autotuner.
+
+For every level of a problem and every tunable operator (SpMV, the four
+SymGS sweep kernels, the fused restriction) and both precisions, time each
+compiled launch shape -- rows per thread {1, 2, 4} x threads per block
+{64, 128, 256} -- with HIP events on the solver stream and keep the fastest.
+Results are cached as JSON keyed by the level's grid and the precision, so a
+later run on the same grids applies them without timing.  The reference has
+no tuner; it hard-codes 1024/256/128-thread blocks (ComputeSPMV.cpp:262,
+ComputeSYMGS.cpp:395, ComputeRestriction.cpp:214).
+"""
+from __future__ import annotations
+
+import json
+import os
+
+from . import (OP_BWD_FULL, OP_BWD_ZERO, OP_FWD_FULL, OP_FWD_ZERO, OP_RESTRICT, OP_SPMV,
+               SH_BWD_FULL, SH_BWD_ZERO, SH_FWD_FULL, SH_FWD_ZERO, SH_RESTRICT, SH_SPMV,
+               STORE_F64, STORE_MIXED)
+
+SHAPES = [(block, rows) for rows in (1, 2, 4) for block in (64, 128, 256)]
+OPS = {  # tunable operator -> timing operator
+    SH_SPMV: OP_SPMV, SH_FWD_ZERO: OP_FWD_ZERO, SH_BWD_ZERO: OP_BWD_ZERO,
+    SH_FWD_FULL: OP_FWD_FULL, SH_BWD_FULL: OP_BWD_FULL, SH_RESTRICT: OP_RESTRICT,
+}
+OP_NAMES = {SH_SPMV: "spmv", SH_FWD_ZERO: "fwd_zero", SH_BWD_ZERO: "bwd_zero",
+            SH_FWD_FULL: "fwd", SH_BWD_FULL: "bwd", SH_RESTRICT: "restrict"}
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "rochpcg_amd", "shapes.json")
+
+
+def _key(level, mixed):
+    return f"{level.nx}x{level.ny}x{level.nz}:{'mixed' if mixed else 'f64'}"
+
+
+def load(path=DEFAULT_CACHE):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save(table, path=DEFAULT_CACHE):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def apply(prob, table):
+    """Apply cached shapes to every level/precision found in `table`; returns count."""
+    n = 0
+    for lv, L in enumerate(prob.levels):
+        for mixed in (False, True):
+            if not (L.stores & (STORE_MIXED if mixed else STORE_F64)):
+                continue
+            entry = table.get(_key(L, mixed))
+            if not entry:
+                continue
+            for op, name in OP_NAMES.items():
+                if name in entry:
+                    block, rows = entry[name]["shape"]
+                    prob.set_shape(block, rows, level=lv, op=op, mixed=mixed)
+                    n += 1
+    return n
+
+
+def tune(prob, mixed=False, reps=5, levels=None, table=None, log=None):
+    """Time every shape of every tunable operator; apply and return the table."""
+    table = {} if table is None else table
+    last = prob.nlevels - 1
+    for lv in (range(prob.nlevels) if levels is None else levels):
+        L = prob.levels[lv]
+        if not (L.stores & (STORE_MIXED if mixed else STORE_F64)):
+            continue
+        entry = table.setdefault(_key(L, mixed), {})
+        for op, top in OPS.items():
+            if op == SH_RESTRICT and lv == last:
+                continue
+            cands = SHAPES if op != SH_RESTRICT else [(b, 1) for b in (64, 128, 256)]
+            timings = {}
+            for block, rows in cands:
+                prob.set_shape(block, rows, level=lv, op=op, mixed=mixed)
+                timings[(block, rows)] = prob.time_operator(top, level=lv, mixed=mixed, reps=reps)
+            (block, rows), best = min(timings.items(), key=lambda kv: kv[1])
+            prob.set_shape(block, rows, level=lv, op=op, mixed=mixed)
+            default = timings.get((256, 1))
+            entry[OP_NAMES[op]] = {"shape": [block, rows], "ms": round(best, 5),
+                                   "default_ms": None if default is None else round(default, 5)}
+            if log:
+                log(f"tune {_key(L, mixed)} {OP_NAMES[op]:9s} best block={block} rows={rows} "
+                    f"{best * 1e3:9.1f} us (256x1: {default * 1e3:9.1f} us)")
+    return table

@@ -229,6 +229,7 @@ static RedArgs store_red(Problem* P)
     ra.ticket = P->ticket;
     ra.op = OP_STORE;
     ra.out = P->dout;
+    ra.cap = P->partial_cap;
     return ra;
 }
 
@@ -335,48 +336,47 @@ double hpcgx_flops_per_iteration(const hpcgx_problem* p)
 }
 
 // ---------------------------------------------------------------------------
-int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int reps, double* avg_ms)
+int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps, double* avg_ms)
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
     Problem* P = p->P;
     const Level& L = P->L[level];
-    const bool need64 = (op == 0 || op == 1 || op == 2 || op == 5);
-    const bool need32 = (op == 3 || op == 4 || op == 6);
-    if((need64 && !P->has_f64) || (need32 && !P->has_mixed) || op < 0 || op > 6 || reps < 1)
+    const bool mixed = prec != 0;
+    if((mixed && !P->has_mixed) || (!mixed && !P->has_f64) || op < 0 || op > 8 || reps < 1)
     {
         set_error("operator not available for this problem");
         return -2;
     }
-    if((op == 5 || op == 6) && level != 0)
+    if(op == 8 && level + 1 >= P->nlevels)
     {
-        set_error("MG timing starts at level 0");
+        set_error("the coarsest level has no restriction");
         return -2;
     }
-    // operands: level-sized scratch that the CG owns (contents irrelevant)
-    double* a64 = (level == 0) ? P->p : (double*)P->v64[level].r;
-    double* b64 = (level == 0) ? P->Ap : (double*)P->v64[level].x;
-    if(!P->has_f64 && level > 0)
-    {
-        a64 = P->p;
-        b64 = P->Ap;
-    }
+    // operands as the V-cycle uses them: level 0 takes the CG residual (f64)
+    // and writes z; coarser levels use their own r / x buffers
+    const Prec pr = mixed ? PREC_MIXED : PREC_F64;
+    LevelVecs* V = mixed ? P->v32 : P->v64;
+    const void* r = (level == 0) ? (const void*)P->r : V[level].r;
+    void* x = (level == 0) ? (mixed ? (void*)P->z32 : (void*)P->z64) : V[level].x;
+    const bool r_f64 = (level == 0) || !mixed;
     auto run = [&]() -> int {
         switch(op)
         {
-        case 0: return launch_spmv_f64x(L, PREC_F64, a64, b64, nullptr, P->s, nullptr);
-        case 1: return launch_symgs(L, PREC_F64, a64, true, b64, P->v64[level].t, P->s, nullptr);
-        case 2: return launch_symgs_zero(L, PREC_F64, a64, true, b64, P->s, nullptr);
-        case 3: return launch_spmv_f64x(L, PREC_MIXED, P->p, P->Ap, nullptr, P->s, nullptr);
-        case 4:
-            return launch_symgs(L, PREC_MIXED, level == 0 ? (void*)P->r : P->v32[level].r, level == 0,
-                                level == 0 ? (void*)P->z32 : P->v32[level].x, P->v32[level].t, P->s, nullptr);
-        case 5: return mg(*P, 0, PREC_F64, P->r, true, P->z64, nullptr);
-        case 6: return mg(*P, 0, PREC_MIXED, P->r, true, P->z32, nullptr);
+        case 0:
+            if(mixed) return launch_spmv_f64x(L, PREC_MIXED, P->p, P->Ap, nullptr, P->s, nullptr);
+            return launch_spmv_f64x(L, PREC_F64, (level == 0) ? P->p : (const double*)V[level].r,
+                                    (level == 0) ? P->Ap : (double*)V[level].x, nullptr, P->s, nullptr);
+        case 1: return launch_symgs(L, pr, r, r_f64, x, V[level].t, P->s, nullptr);
+        case 2: return launch_symgs_zero(L, pr, r, r_f64, x, P->s, nullptr);
+        case 3: return mg(*P, level, pr, r, r_f64, x, nullptr);
+        case 4: case 5: case 6: case 7:
+            return launch_sweep(L, pr, op - 4, r, r_f64, x, V[level].t, P->s, nullptr);
+        case 8: return launch_restrict(L, pr, r, r_f64, x, V[level + 1].r, P->s, nullptr);
         }
         return -2;
     };
-    HPCGX_TRY(run());   // warm-up
+    HPCGX_TRY(run());   // warm-up (and first-launch costs)
     hipEvent_t e0, e1;
     HPCGX_HIP(hipEventCreate(&e0));
     HPCGX_HIP(hipEventCreate(&e1));
@@ -389,6 +389,43 @@ int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int reps, double* a
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *avg_ms = ms / reps;
+    return 0;
+}
+
+int hpcgx_set_shape(hpcgx_problem* p, int prec, int level, int op, int block, int rows)
+{
+    CHECK_P(p);
+    if(prec < 0 || prec > 1 || level < -1 || level >= p->P->nlevels || op < -1 || op >= SH_COUNT ||
+       !shape_supported(rows, block))
+    {
+        set_error("invalid precision, level, operator or launch shape");
+        return -2;
+    }
+    for(int l = 0; l < p->P->nlevels; ++l)
+    {
+        if(level >= 0 && l != level) continue;
+        for(int o = 0; o < SH_COUNT; ++o)
+        {
+            if(op >= 0 && o != op) continue;
+            p->P->L[l].shp[prec][o].block = block;
+            p->P->L[l].shp[prec][o].rows = (o == SH_RESTRICT) ? 1 : rows;
+        }
+    }
+    p->P->invalidate_graphs();
+    return 0;
+}
+
+int hpcgx_get_shape(const hpcgx_problem* p, int prec, int level, int op, int* block, int* rows)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    if(op < 0 || op >= SH_COUNT || prec < 0 || prec > 1)
+    {
+        set_error("invalid precision or operator");
+        return -2;
+    }
+    *block = p->P->L[level].shp[prec][op].block;
+    *rows = p->P->L[level].shp[prec][op].rows;
     return 0;
 }
 

@@ -53,7 +53,20 @@ enum Prec : int
 // Launch shape of one operator on one level (chosen by the autotuner).
 struct Shape
 {
-    int block = 256;   // threads per workgroup (multiple of 64)
+    int block = 256;   // threads per workgroup: 64, 128 or 256
+    int rows = 1;      // consecutive rows per thread: 1, 2 or 4
+};
+
+// tunable operators of a level
+enum ShapeOp : int
+{
+    SH_SPMV = 0,
+    SH_FWD_ZERO = 1,
+    SH_BWD_ZERO = 2,
+    SH_FWD_FULL = 3,
+    SH_BWD_FULL = 4,
+    SH_RESTRICT = 5,
+    SH_COUNT = 6
 };
 
 // One level of the hierarchy, device pointers plus host-side colour table.
@@ -85,7 +98,7 @@ struct Level
     int32_t* f2c_coarse = nullptr;  // [mc] multicolour coarse row of k
     int32_t* f2c_nat = nullptr;     // [mc] natural fine row (reference f2cOperator)
 
-    Shape spmv, sweep, restrict_, prolong;
+    Shape shp[2][SH_COUNT];   // [precision][operator]
 };
 
 // Device-side CG scalars (one cache line each to avoid false sharing is not

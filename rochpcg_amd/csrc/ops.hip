@@ -19,6 +19,8 @@
 #include "common.hpp"
 #include "ops.hpp"
 
+#include <type_traits>
+
 namespace hpcgx
 {
 
@@ -114,41 +116,119 @@ __device__ __forceinline__ void reduce_finish(double v, const RedArgs& ra)
 }
 
 // ---------------------------------------------------------------------------
+// Vector loads of R consecutive rows of one ELL slot.  A thread owns rows
+// [row, row+R) (row % R == 0, m % R == 0), so a wave issues one contiguous
+// 64*R-element segment per slot: dwordx2/x4 loads instead of R dword loads.
+template <typename T, int R>
+struct Ext;
+#define HPCGX_EXT(T, R) \
+    template <>         \
+    struct Ext<T, R>    \
+    {                   \
+        typedef T type __attribute__((ext_vector_type(R))); \
+    };
+HPCGX_EXT(int32_t, 2)
+HPCGX_EXT(int32_t, 4)
+HPCGX_EXT(double, 2)
+HPCGX_EXT(double, 4)
+HPCGX_EXT(float, 2)
+HPCGX_EXT(float, 4)
+HPCGX_EXT(uint16_t, 2)
+HPCGX_EXT(uint16_t, 4)
+#undef HPCGX_EXT
+
+template <int R, typename T>
+__device__ __forceinline__ void ld_nt(const T* p, T (&o)[R])
+{
+    if constexpr(R == 1)
+        o[0] = ntload(p);
+    else
+    {
+        typedef typename Ext<T, R>::type V;
+        const V w = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+#pragma unroll
+        for(int k = 0; k < R; ++k) o[k] = w[k];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void ld_nt(const __half* p, __half (&o)[R])
+{
+    if constexpr(R == 1)
+        o[0] = ntload(p);
+    else
+    {
+        typedef typename Ext<uint16_t, R>::type V;
+        const V w = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+#pragma unroll
+        for(int k = 0; k < R; ++k) o[k] = __ushort_as_half(w[k]);
+    }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void ld(const T* p, T (&o)[R])
+{
+    if constexpr(R == 1)
+        o[0] = *p;
+    else
+    {
+        typedef typename Ext<T, R>::type V;
+        const V w = *reinterpret_cast<const V*>(p);
+#pragma unroll
+        for(int k = 0; k < R; ++k) o[k] = w[k];
+    }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void st(T* p, const T (&v)[R])
+{
+    if constexpr(R == 1)
+        *p = v[0];
+    else
+    {
+        typedef typename Ext<T, R>::type V;
+        V w;
+#pragma unroll
+        for(int k = 0; k < R; ++k) w[k] = v[k];
+        *reinterpret_cast<V*>(p) = w;
+    }
+}
+
+// r may be f64 while x is f32 (mixed level 0): load in RT, convert.
+template <int R, typename XT, typename RT>
+__device__ __forceinline__ void ld_cvt(const RT* p, XT (&o)[R])
+{
+    RT t[R];
+    ld<R>(p, t);
+#pragma unroll
+    for(int k = 0; k < R; ++k) o[k] = (XT)t[k];
+}
+
+template <int R, typename XT, typename YT>
+__device__ __forceinline__ void st_cvt(YT* p, const XT (&v)[R])
+{
+    YT t[R];
+#pragma unroll
+    for(int k = 0; k < R; ++k) t[k] = (YT)v[k];
+    st<R>(p, t);
+}
+
+// ---------------------------------------------------------------------------
 // Row tiles of a whole-level operator: tile t -> (slab j = t / ncolors,
-// colour c = t % ncolors), rows offsets[c] + j*BS + [0, BS) clipped to the
-// colour.  Consecutive tiles therefore cover the same spatial slab in every
-// colour, and each block takes a contiguous run of `tpb` tiles of its XCD's
-// slice.
+// colour c = t % ncolors), rows offsets[c] + j*BS*R + [0, BS*R) clipped to the
+// colour.  Consecutive tiles cover the same spatial slab in every colour, and
+// consecutive blocks of an XCD take consecutive runs of `tpb` tiles, so the
+// x gathers of the rows in flight stay inside a compact slab.
 struct Tiles
 {
     int ncolors;
-    int jtiles;   // max over colours of ceil(size/BS)
+    int jtiles;   // max over colours of ceil(size / (BS*R))
     int tpb;      // tiles per block
     int off[MAX_COLORS];
     int size[MAX_COLORS];
 };
 
-// Row loads are issued in phases -- all slot indices and values of the row
-// first (independent, non-temporal), then the x gathers of the valid slots,
-// then the fma chain in ascending slot order -- so one row keeps 27+27+27
-// requests in flight instead of serialising slot by slot.
-template <typename VT, typename XT>
-__device__ __forceinline__ void load_row(const int32_t* cp, const VT* vp, int64_t m, int32_t (&cc)[W], XT (&a)[W])
-{
-#pragma unroll
-    for(int p = 0; p < W; ++p) cc[p] = ntload(cp + (int64_t)p * m);
-#pragma unroll
-    for(int p = 0; p < W; ++p) a[p] = cvt<XT>(ntload(vp + (int64_t)p * m));
-}
-
-template <typename XT>
-__device__ __forceinline__ void gather(const XT* __restrict__ x, const int32_t (&cc)[W], XT (&xv)[W])
-{
-#pragma unroll
-    for(int p = 0; p < W; ++p) xv[p] = (cc[p] >= 0) ? x[cc[p]] : (XT)0;
-}
-
-template <typename VT, typename XT, typename YT, bool DOT, int BS>
+template <typename VT, typename XT, typename YT, bool DOT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restrict__ col,
                                             const VT* __restrict__ val, const XT* __restrict__ x,
                                             YT* __restrict__ y, Tiles tl, RedArgs ra, const int* gate)
@@ -162,146 +242,238 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
     for(int64_t t = t0; t < t1; ++t)
     {
         const int c = (int)(t % tl.ncolors);
-        const int64_t i = (t / tl.ncolors) * BS + threadIdx.x;
+        const int64_t i = (t / tl.ncolors) * (BS * R) + (int64_t)threadIdx.x * R;
         if(i >= tl.size[c]) continue;
         const int64_t row = tl.off[c] + i;
-        int32_t cc[W];
-        XT a[W], xv[W];
-        load_row(col + row, val + row, m, cc, a);
-        gather(x, cc, xv);
-        XT s = 0;
+        int32_t cc[W][R];
+        VT av[W][R];
+#pragma unroll
+        for(int p = 0; p < W; ++p) ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
+#pragma unroll
+        for(int p = 0; p < W; ++p) ld_nt<R>(val + (int64_t)p * m + row, av[p]);
+        XT xv[W][R];
 #pragma unroll
         for(int p = 0; p < W; ++p)
-            if(cc[p] >= 0) s = fmaT(a[p], xv[p], s);
-        y[row] = (YT)s;
-        if(DOT) acc = fmaT((double)x[row], (double)s, acc);
+#pragma unroll
+            for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
+        XT s[R];
+#pragma unroll
+        for(int k = 0; k < R; ++k) s[k] = 0;
+#pragma unroll
+        for(int p = 0; p < W; ++p)
+#pragma unroll
+            for(int k = 0; k < R; ++k)
+                if(cc[p][k] >= 0) s[k] = fmaT(cvt<XT>(av[p][k]), xv[p][k], s[k]);
+        st_cvt<R>(y + row, s);
+        if(DOT)
+        {
+            XT xr[R];
+            ld<R>(x + row, xr);
+#pragma unroll
+            for(int k = 0; k < R; ++k) acc = fmaT((double)xr[k], (double)s[k], acc);
+        }
     }
     if(DOT) reduce_finish<BS>(acc, ra);
 }
 
+// Per-colour sweeps: block b covers rows off + xcd_block(b)*BS*R + [0, BS*R).
+#define SWEEP_ROW()                                                                   \
+    if(gated(gate)) return;                                                           \
+    const int64_t i = xcd_block(blockIdx.x, gridDim.x) * (BS * R) + (int64_t)threadIdx.x * R; \
+    if(i >= size) return;                                                             \
+    const int64_t row = off + i;
+
 // Zero-initial-guess forward sweep of colour c: only entries left of the
 // diagonal (earlier colours) are non-zero in x.
-template <typename VT, typename XT, typename RT, int BS>
+template <typename VT, typename XT, typename RT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ dgi,
                                                  const RT* __restrict__ r, XT* __restrict__ x, const int* gate)
 {
-    if(gated(gate)) return;
-    const int64_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    if(i >= size) return;
-    const int64_t row = off + i;
-    const int dg = dgi[row];
-    const int32_t* cp = col + row;
-    const VT* vp = val + row;
-    int32_t cc[W];
-    XT a[W], xv[W];
+    SWEEP_ROW();
+    int32_t dg[R];
+    ld<R>(dgi + row, dg);
+    int dmax = dg[0];
 #pragma unroll
-    for(int p = 0; p < W; ++p) cc[p] = (p < dg) ? ntload(cp + (int64_t)p * m) : -1;
-#pragma unroll
-    for(int p = 0; p < W; ++p) a[p] = (p < dg) ? cvt<XT>(ntload(vp + (int64_t)p * m)) : (XT)0;
-    const XT d = cvt<XT>(vp[(int64_t)dg * m]);
-    XT s = cvt<XT>(r[row]);
-    gather(x, cc, xv);
+    for(int k = 1; k < R; ++k) dmax = dg[k] > dmax ? dg[k] : dmax;
+    int32_t cc[W][R];
+    VT av[W][R];
 #pragma unroll
     for(int p = 0; p < W; ++p)
-        if(p < dg) s = fmaT(-a[p], xv[p], s);
-    x[row] = s / d;
+    {
+        if(p < dmax)
+        {
+            ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
+            ld_nt<R>(val + (int64_t)p * m + row, av[p]);
+        }
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            if(!(p < dg[k])) cc[p][k] = -1;
+    }
+    XT d[R], s[R];
+#pragma unroll
+    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)dg[k] * m + row + k]);
+    ld_cvt<R>(r + row, s);
+    XT xv[W][R];
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+#pragma unroll
+        for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            if(cc[p][k] >= 0) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+#pragma unroll
+    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
+    st<R>(x + row, s);
 }
 
 // Zero-initial-guess backward sweep of colour c: x = (x*d - U x) / d.
-template <typename VT, typename XT, int BS>
+template <typename VT, typename XT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_bwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ dgi,
                                                  XT* __restrict__ x, const int* gate)
 {
-    if(gated(gate)) return;
-    const int64_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    if(i >= size) return;
-    const int64_t row = off + i;
-    const int dg = dgi[row];
-    const int32_t* cp = col + row;
-    const VT* vp = val + row;
-    int32_t cc[W];
-    XT a[W], xv[W];
+    SWEEP_ROW();
+    int32_t dg[R];
+    ld<R>(dgi + row, dg);
+    int dmin = dg[0];
 #pragma unroll
-    for(int p = 0; p < W; ++p) cc[p] = (p > dg) ? ntload(cp + (int64_t)p * m) : -1;
-#pragma unroll
-    for(int p = 0; p < W; ++p) a[p] = (p > dg) ? cvt<XT>(ntload(vp + (int64_t)p * m)) : (XT)0;
-    const XT d = cvt<XT>(vp[(int64_t)dg * m]);
-    XT s = x[row] * d;
-    gather(x, cc, xv);
+    for(int k = 1; k < R; ++k) dmin = dg[k] < dmin ? dg[k] : dmin;
+    int32_t cc[W][R];
+    VT av[W][R];
 #pragma unroll
     for(int p = 0; p < W; ++p)
-        if(cc[p] >= 0) s = fmaT(-a[p], xv[p], s);
-    x[row] = s / d;
+    {
+        if(p > dmin)
+        {
+            ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
+            ld_nt<R>(val + (int64_t)p * m + row, av[p]);
+        }
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            if(!(p > dg[k])) cc[p][k] = -1;
+    }
+    XT d[R], s[R];
+#pragma unroll
+    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)dg[k] * m + row + k]);
+    ld<R>(x + row, s);
+#pragma unroll
+    for(int k = 0; k < R; ++k) s[k] = s[k] * d[k];
+    XT xv[W][R];
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+#pragma unroll
+        for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            if(cc[p][k] >= 0) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+#pragma unroll
+    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
+    st<R>(x + row, s);
 }
 
 // Full forward sweep of colour c over the whole row, in ascending column
 // order; keeps t = r - L x (the sum up to the diagonal) for the backward
 // sweep, whose lower neighbours do not change in between.
-template <typename VT, typename XT, typename RT, bool STORE_T, int BS>
+template <typename VT, typename XT, typename RT, bool STORE_T, int R, int BS>
 __global__ __launch_bounds__(BS) void k_fwd_full(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const RT* __restrict__ r,
                                                  XT* __restrict__ x, XT* __restrict__ t, const int* gate)
 {
-    if(gated(gate)) return;
-    const int64_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    if(i >= size) return;
-    const int64_t row = off + i;
-    int32_t cc[W];
-    XT a[W], xv[W];
-    load_row(col + row, val + row, m, cc, a);
+    SWEEP_ROW();
+    int32_t cc[W][R];
+    VT av[W][R];
 #pragma unroll
-    for(int p = 0; p < W; ++p) xv[p] = (cc[p] >= 0 && (int64_t)cc[p] != row) ? x[cc[p]] : (XT)0;
-    XT s = cvt<XT>(r[row]);
-    XT tt = 0, d = 1;
+    for(int p = 0; p < W; ++p) ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
+#pragma unroll
+    for(int p = 0; p < W; ++p) ld_nt<R>(val + (int64_t)p * m + row, av[p]);
+    XT xv[W][R];
 #pragma unroll
     for(int p = 0; p < W; ++p)
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            xv[p][k] = (cc[p][k] >= 0 && (int64_t)cc[p][k] != row + k) ? x[cc[p][k]] : (XT)0;
+    XT s[R], tt[R], d[R];
+    ld_cvt<R>(r + row, s);
+#pragma unroll
+    for(int k = 0; k < R; ++k)
     {
-        if(cc[p] >= 0)
+        tt[k] = 0;
+        d[k] = 1;
+    }
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+#pragma unroll
+        for(int k = 0; k < R; ++k)
         {
-            if((int64_t)cc[p] == row)
+            if(cc[p][k] >= 0)
             {
-                tt = s;
-                d = a[p];
-            }
-            else
-            {
-                s = fmaT(-a[p], xv[p], s);
+                if((int64_t)cc[p][k] == row + k)
+                {
+                    tt[k] = s[k];
+                    d[k] = cvt<XT>(av[p][k]);
+                }
+                else
+                {
+                    s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+                }
             }
         }
-    }
-    x[row] = s / d;
-    if(STORE_T) t[row] = tt;
+#pragma unroll
+    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
+    st<R>(x + row, s);
+    if(STORE_T) st<R>(t + row, tt);
 }
 
 // Full backward sweep of colour c using t: x = (t - U x) / d.
-template <typename VT, typename XT, int BS>
+template <typename VT, typename XT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_bwd_full(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ dgi,
                                                  const XT* __restrict__ t, XT* __restrict__ x, const int* gate)
 {
-    if(gated(gate)) return;
-    const int64_t i = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    if(i >= size) return;
-    const int64_t row = off + i;
-    const int dg = dgi[row];
-    const int32_t* cp = col + row;
-    const VT* vp = val + row;
-    int32_t cc[W];
-    XT a[W], xv[W];
+    SWEEP_ROW();
+    int32_t dg[R];
+    ld<R>(dgi + row, dg);
+    int dmin = dg[0];
 #pragma unroll
-    for(int p = 0; p < W; ++p) cc[p] = (p > dg) ? ntload(cp + (int64_t)p * m) : -1;
-#pragma unroll
-    for(int p = 0; p < W; ++p) a[p] = (p > dg) ? cvt<XT>(ntload(vp + (int64_t)p * m)) : (XT)0;
-    const XT d = cvt<XT>(vp[(int64_t)dg * m]);
-    XT s = t[row];
-    gather(x, cc, xv);
+    for(int k = 1; k < R; ++k) dmin = dg[k] < dmin ? dg[k] : dmin;
+    int32_t cc[W][R];
+    VT av[W][R];
 #pragma unroll
     for(int p = 0; p < W; ++p)
-        if(cc[p] >= 0) s = fmaT(-a[p], xv[p], s);
-    x[row] = s / d;
+    {
+        if(p > dmin)
+        {
+            ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
+            ld_nt<R>(val + (int64_t)p * m + row, av[p]);
+        }
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            if(!(p > dg[k])) cc[p][k] = -1;
+    }
+    XT d[R], s[R];
+#pragma unroll
+    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)dg[k] * m + row + k]);
+    ld<R>(t + row, s);
+    XT xv[W][R];
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+#pragma unroll
+        for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            if(cc[p][k] >= 0) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+#pragma unroll
+    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
+    st<R>(x + row, s);
 }
+#undef SWEEP_ROW
 
 // Fused fine residual + injection for coarse natural row k:
 // rc[f2c_coarse[k]] = r[f] - sum_p a[f,p] x[col], f = f2c_fine[k].
@@ -315,14 +487,19 @@ __global__ __launch_bounds__(BS) void k_restrict(int64_t m, int64_t mc, const in
     const int64_t k = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if(k >= mc) return;
     const int64_t row = f2c_fine[k];
-    int32_t cc[W];
-    XT a[W], xv[W];
-    load_row(col + row, val + row, m, cc, a);
-    gather(x, cc, xv);
+    int32_t cc[W][1];
+    VT av[W][1];
+#pragma unroll
+    for(int p = 0; p < W; ++p) ld_nt<1>(col + (int64_t)p * m + row, cc[p]);
+#pragma unroll
+    for(int p = 0; p < W; ++p) ld_nt<1>(val + (int64_t)p * m + row, av[p]);
+    XT xv[W];
+#pragma unroll
+    for(int p = 0; p < W; ++p) xv[p] = (cc[p][0] >= 0) ? x[cc[p][0]] : (XT)0;
     XT s = cvt<XT>(r[row]);
 #pragma unroll
     for(int p = 0; p < W; ++p)
-        if(cc[p] >= 0) s = fmaT(-a[p], xv[p], s);
+        if(cc[p][0] >= 0) s = fmaT(-cvt<XT>(av[p][0]), xv[p], s);
     rc[f2c_coarse[k]] = (CT)s;
 }
 
@@ -486,7 +663,43 @@ static inline unsigned nblk(int64_t n, int bs) { return (unsigned)ceil_div(n, bs
 
 #define HPCGX_KCHECK() HPCGX_HIP(hipGetLastError())
 
-static Tiles make_tiles(const Level& L, int bs, int max_blocks, int64_t& nblocks)
+// Launch shapes compiled in: rows per thread {1,2,4} x block {64,128,256}.
+template <typename F>
+static int with_shape(const Shape& sh, F&& f)
+{
+    using std::integral_constant;
+    switch(sh.rows * 1000 + sh.block)
+    {
+    case 1064: return f(integral_constant<int, 1>{}, integral_constant<int, 64>{});
+    case 1128: return f(integral_constant<int, 1>{}, integral_constant<int, 128>{});
+    case 1256: return f(integral_constant<int, 1>{}, integral_constant<int, 256>{});
+    case 2064: return f(integral_constant<int, 2>{}, integral_constant<int, 64>{});
+    case 2128: return f(integral_constant<int, 2>{}, integral_constant<int, 128>{});
+    case 2256: return f(integral_constant<int, 2>{}, integral_constant<int, 256>{});
+    case 4064: return f(integral_constant<int, 4>{}, integral_constant<int, 64>{});
+    case 4128: return f(integral_constant<int, 4>{}, integral_constant<int, 128>{});
+    case 4256: return f(integral_constant<int, 4>{}, integral_constant<int, 256>{});
+    default: break;
+    }
+    set_error("unsupported launch shape (rows " + std::to_string(sh.rows) + ", block " + std::to_string(sh.block) + ")");
+    return -2;
+}
+
+bool shape_supported(int rows, int block)
+{
+    return (rows == 1 || rows == 2 || rows == 4) && (block == 64 || block == 128 || block == 256);
+}
+
+// R rows per thread need every colour block (and m) to be R-aligned.
+static Shape fit(const Level& L, Shape sh)
+{
+    bool ok = (L.m % sh.rows) == 0;
+    for(int c = 0; c < L.ncolors && ok; ++c) ok = (L.offsets[c] % sh.rows) == 0 && (L.sizes[c] % sh.rows) == 0;
+    if(!ok) sh.rows = 1;
+    return sh;
+}
+
+static Tiles make_tiles(const Level& L, int rows_per_tile, bool dot, int64_t cap, int64_t& nblocks)
 {
     Tiles t{};
     t.ncolors = L.ncolors;
@@ -497,10 +710,12 @@ static Tiles make_tiles(const Level& L, int bs, int max_blocks, int64_t& nblocks
         t.size[c] = L.sizes[c];
         mx = mx > L.sizes[c] ? mx : L.sizes[c];
     }
-    t.jtiles = (int)ceil_div(mx, bs);
+    t.jtiles = (int)ceil_div(mx, rows_per_tile);
     const int64_t ntiles = (int64_t)t.ncolors * t.jtiles;
-    t.tpb = (max_blocks > 0) ? (int)ceil_div(ntiles, max_blocks) : 1;
-    if(t.tpb < 1) t.tpb = 1;
+    // one slab of every colour per block keeps the partials few and the
+    // blocks of an XCD on neighbouring slabs
+    t.tpb = dot ? t.ncolors : 1;
+    while(dot && ceil_div(ntiles, t.tpb) > cap) t.tpb += t.ncolors;
     nblocks = ceil_div(ntiles, t.tpb);
     return t;
 }
@@ -516,17 +731,20 @@ const __half* vals_of<__half>(const Level& L) { return L.ell_h; }
 template <typename VT, typename XT, typename YT>
 static int spmv_t(const Level& L, const void* x, void* y, const RedArgs* ra, hipStream_t s, const int* gate)
 {
-    constexpr int BS = 256;
-    int64_t nb;
-    Tiles tl = make_tiles(L, BS, ra ? RED_BLOCKS : 0, nb);
-    if(ra)
-        hipLaunchKernelGGL((k_spmv<VT, XT, YT, true, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m, L.ell_col,
-                           vals_of<VT>(L), (const XT*)x, (YT*)y, tl, *ra, gate);
-    else
-        hipLaunchKernelGGL((k_spmv<VT, XT, YT, false, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m, L.ell_col,
-                           vals_of<VT>(L), (const XT*)x, (YT*)y, tl, RedArgs{}, gate);
-    HPCGX_KCHECK();
-    return 0;
+    const Shape sh = fit(L, L.shp[std::is_same<VT, double>::value ? PREC_F64 : PREC_MIXED][SH_SPMV]);
+    return with_shape(sh, [&](auto RR, auto BB) -> int {
+        constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
+        int64_t nb;
+        Tiles tl = make_tiles(L, BS * R, ra != nullptr, ra ? ra->cap : 0, nb);
+        if(ra)
+            hipLaunchKernelGGL((k_spmv<VT, XT, YT, true, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m, L.ell_col,
+                               vals_of<VT>(L), (const XT*)x, (YT*)y, tl, *ra, gate);
+        else
+            hipLaunchKernelGGL((k_spmv<VT, XT, YT, false, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m,
+                               L.ell_col, vals_of<VT>(L), (const XT*)x, (YT*)y, tl, RedArgs{}, gate);
+        HPCGX_KCHECK();
+        return 0;
+    });
 }
 
 int launch_spmv(const Level& L, Prec prec, const void* x, void* y, hipStream_t s, const int* gate)
@@ -543,85 +761,109 @@ int launch_spmv_f64x(const Level& L, Prec prec, const double* x, double* y, cons
     return spmv_t<__half, double, double>(L, x, y, ra, s, gate);
 }
 
-template <typename VT, typename XT, typename RT>
-static int symgs_zero_t(const Level& L, const void* rv, void* xv, hipStream_t s, const int* gate)
+// one launch per colour c in [c0, c1) (step +1) or (c0, c1] (step -1)
+template <typename Launch>
+static int per_color(const Level& L, int prec, int which, int c_first, int c_last, int step, Launch&& launch)
 {
-    constexpr int BS = 256;
-    const RT* r = (const RT*)rv;
-    XT* x = (XT*)xv;
-    const VT* val = vals_of<VT>(L);
-    for(int c = 0; c < L.ncolors; ++c)
-    {
-        if(L.sizes[c] == 0) continue;
-        hipLaunchKernelGGL((k_fwd_zero<VT, XT, RT, BS>), dim3(nblk(L.sizes[c], BS)), dim3(BS), 0, s, L.m,
-                           (int64_t)L.offsets[c], (int64_t)L.sizes[c], L.ell_col, val, L.diag_idx, r, x, gate);
-        HPCGX_KCHECK();
-    }
-    for(int c = L.ncolors - 2; c >= 0; --c)
-    {
-        if(L.sizes[c] == 0) continue;
-        hipLaunchKernelGGL((k_bwd_zero<VT, XT, BS>), dim3(nblk(L.sizes[c], BS)), dim3(BS), 0, s, L.m,
-                           (int64_t)L.offsets[c], (int64_t)L.sizes[c], L.ell_col, val, L.diag_idx, x, gate);
-        HPCGX_KCHECK();
-    }
-    return 0;
+    const Shape sh = fit(L, L.shp[prec][which]);
+    return with_shape(sh, [&](auto RR, auto BB) -> int {
+        constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
+        for(int c = c_first; step > 0 ? c <= c_last : c >= c_last; c += step)
+        {
+            if(L.sizes[c] == 0) continue;
+            const dim3 grid(nblk(ceil_div(L.sizes[c], R), BS));
+            launch(std::integral_constant<int, R>{}, std::integral_constant<int, BS>{}, grid, c);
+            HPCGX_KCHECK();
+        }
+        return 0;
+    });
 }
 
+// The four sweep kernels of a level: part 0 = zero-guess forward (colours
+// 0..n-1), 1 = zero-guess backward (n-2..0), 2 = forward keeping t (0..n-1),
+// 3 = backward from t (n-2..0).
 template <typename VT, typename XT, typename RT>
-static int symgs_full_t(const Level& L, const void* rv, void* xv, void* tv, hipStream_t s, const int* gate)
+static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv, hipStream_t s, const int* gate)
 {
-    constexpr int BS = 256;
     const RT* r = (const RT*)rv;
     XT* x = (XT*)xv;
     XT* t = (XT*)tv;
     const VT* val = vals_of<VT>(L);
     const int last = L.ncolors - 1;
-    for(int c = 0; c < L.ncolors; ++c)
+    const int pr = std::is_same<VT, double>::value ? PREC_F64 : PREC_MIXED;
+    switch(part)
     {
-        if(L.sizes[c] == 0) continue;
-        if(c < last)
-            hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, true, BS>), dim3(nblk(L.sizes[c], BS)), dim3(BS), 0, s, L.m,
-                               (int64_t)L.offsets[c], (int64_t)L.sizes[c], L.ell_col, val, r, x, t, gate);
-        else
-            hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, false, BS>), dim3(nblk(L.sizes[c], BS)), dim3(BS), 0, s,
-                               L.m, (int64_t)L.offsets[c], (int64_t)L.sizes[c], L.ell_col, val, r, x, t, gate);
-        HPCGX_KCHECK();
+    case 0:
+        return per_color(L, pr, SH_FWD_ZERO, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c) {
+            constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
+            hipLaunchKernelGGL((k_fwd_zero<VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
+                               (int64_t)L.sizes[c], L.ell_col, val, L.diag_idx, r, x, gate);
+        });
+    case 1:
+        if(last < 1) return 0;
+        return per_color(L, pr, SH_BWD_ZERO, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c) {
+            constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
+            hipLaunchKernelGGL((k_bwd_zero<VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
+                               (int64_t)L.sizes[c], L.ell_col, val, L.diag_idx, x, gate);
+        });
+    case 2:
+        return per_color(L, pr, SH_FWD_FULL, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c) {
+            constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
+            if(c < last)
+                hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                   (int64_t)L.offsets[c], (int64_t)L.sizes[c], L.ell_col, val, r, x, t, gate);
+            else
+                hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                   (int64_t)L.offsets[c], (int64_t)L.sizes[c], L.ell_col, val, r, x, t, gate);
+        });
+    case 3:
+        if(last < 1) return 0;
+        return per_color(L, pr, SH_BWD_FULL, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c) {
+            constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
+            hipLaunchKernelGGL((k_bwd_full<VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
+                               (int64_t)L.sizes[c], L.ell_col, val, L.diag_idx, (const XT*)t, x, gate);
+        });
     }
-    for(int c = last - 1; c >= 0; --c)
-    {
-        if(L.sizes[c] == 0) continue;
-        hipLaunchKernelGGL((k_bwd_full<VT, XT, BS>), dim3(nblk(L.sizes[c], BS)), dim3(BS), 0, s, L.m,
-                           (int64_t)L.offsets[c], (int64_t)L.sizes[c], L.ell_col, val, L.diag_idx, (const XT*)t, x,
-                           gate);
-        HPCGX_KCHECK();
-    }
-    return 0;
+    set_error("invalid sweep part");
+    return -2;
+}
+
+int launch_sweep(const Level& L, Prec prec, int part, const void* r, bool r_f64, void* x, void* t, hipStream_t s,
+                 const int* gate)
+{
+    if(prec == PREC_F64) return sweep_t<double, double, double>(L, part, r, x, t, s, gate);
+    if(r_f64) return sweep_t<__half, float, double>(L, part, r, x, t, s, gate);
+    return sweep_t<__half, float, float>(L, part, r, x, t, s, gate);
 }
 
 int launch_symgs_zero(const Level& L, Prec prec, const void* r, bool r_f64, void* x, hipStream_t s, const int* gate)
 {
-    if(prec == PREC_F64) return symgs_zero_t<double, double, double>(L, r, x, s, gate);
-    if(r_f64) return symgs_zero_t<__half, float, double>(L, r, x, s, gate);
-    return symgs_zero_t<__half, float, float>(L, r, x, s, gate);
+    HPCGX_TRY(launch_sweep(L, prec, 0, r, r_f64, x, nullptr, s, gate));
+    return launch_sweep(L, prec, 1, r, r_f64, x, nullptr, s, gate);
 }
 
 int launch_symgs(const Level& L, Prec prec, const void* r, bool r_f64, void* x, void* t, hipStream_t s,
                  const int* gate)
 {
-    if(prec == PREC_F64) return symgs_full_t<double, double, double>(L, r, x, t, s, gate);
-    if(r_f64) return symgs_full_t<__half, float, double>(L, r, x, t, s, gate);
-    return symgs_full_t<__half, float, float>(L, r, x, t, s, gate);
+    HPCGX_TRY(launch_sweep(L, prec, 2, r, r_f64, x, t, s, gate));
+    return launch_sweep(L, prec, 3, r, r_f64, x, t, s, gate);
 }
 
 template <typename VT, typename XT, typename RT>
 static int restrict_t(const Level& F, const void* r, const void* x, void* rc, hipStream_t s, const int* gate)
 {
-    constexpr int BS = 256;
-    hipLaunchKernelGGL((k_restrict<VT, XT, RT, XT, BS>), dim3(nblk(F.mc, BS)), dim3(BS), 0, s, F.m, F.mc,
-                       F.ell_col, vals_of<VT>(F), F.f2c_fine, F.f2c_coarse, (const RT*)r, (const XT*)x, (XT*)rc,
-                       gate);
-    HPCGX_KCHECK();
-    return 0;
+    const int bs = F.shp[std::is_same<VT, double>::value ? PREC_F64 : PREC_MIXED][SH_RESTRICT].block;
+    auto go = [&](auto BB) -> int {
+        constexpr int BS = decltype(BB)::value;
+        hipLaunchKernelGGL((k_restrict<VT, XT, RT, XT, BS>), dim3(nblk(F.mc, BS)), dim3(BS), 0, s, F.m, F.mc,
+                           F.ell_col, vals_of<VT>(F), F.f2c_fine, F.f2c_coarse, (const RT*)r, (const XT*)x,
+                           (XT*)rc, gate);
+        HPCGX_KCHECK();
+        return 0;
+    };
+    if(bs == 64) return go(std::integral_constant<int, 64>{});
+    if(bs == 128) return go(std::integral_constant<int, 128>{});
+    return go(std::integral_constant<int, 256>{});
 }
 
 int launch_restrict(const Level& F, Prec prec, const void* r, bool r_f64, const void* x, void* rc, hipStream_t s,

@@ -26,12 +26,18 @@ struct RedArgs
     double* hist = nullptr;
     int hist_cap = -1;
     double* out = nullptr;
+    int64_t cap = RED_BLOCKS;      // partial slots available
 };
+
+bool shape_supported(int rows, int block);
 
 // `gate` (device int, may be null): kernels return at once while *gate != 0.
 int launch_spmv(const Level& L, Prec prec, const void* x, void* y, hipStream_t s, const int* gate);
 int launch_spmv_f64x(const Level& L, Prec prec, const double* x, double* y, const RedArgs* ra, hipStream_t s,
                      const int* gate);
+// one of the four sweep kernels of a level (0 zero-guess fwd, 1 zero-guess bwd, 2 fwd keeping t, 3 bwd from t)
+int launch_sweep(const Level& L, Prec prec, int part, const void* r, bool r_f64, void* x, void* t, hipStream_t s,
+                 const int* gate);
 int launch_symgs_zero(const Level& L, Prec prec, const void* r, bool r_f64, void* x, hipStream_t s, const int* gate);
 int launch_symgs(const Level& L, Prec prec, const void* r, bool r_f64, void* x, void* t, hipStream_t s, const int* gate);
 int launch_restrict(const Level& F, Prec prec, const void* r, bool r_f64, const void* x, void* rc, hipStream_t s,

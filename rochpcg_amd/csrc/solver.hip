@@ -103,8 +103,10 @@ int problem_create(Problem*& out, int nx, int ny, int nz, int nlevels, bool with
     if(with_f64 && alloc((void**)&P->z64, m0 * 8)) return fail(-1);
     if(with_mixed && alloc((void**)&P->z32, m0 * 4)) return fail(-1);
     P->hist_cap = 4096;
+    // fused SpMV + dot keeps one partial per slab of 64 rows at most
+    P->partial_cap = RED_BLOCKS + (int64_t)(m0 / 64) + 64;
     if(alloc((void**)&P->sc, sizeof(CGScalars)) || alloc((void**)&P->hist, sizeof(double) * (P->hist_cap + 1)) ||
-       alloc((void**)&P->partial, sizeof(double) * RED_BLOCKS) || alloc((void**)&P->ticket, 4 * sizeof(unsigned)) ||
+       alloc((void**)&P->partial, sizeof(double) * P->partial_cap) || alloc((void**)&P->ticket, 4 * sizeof(unsigned)) ||
        alloc((void**)&P->dout, 8 * sizeof(double)))
         return fail(-1);
     if(hipMemsetAsync(P->ticket, 0, 4 * sizeof(unsigned), P->s) != hipSuccess ||
@@ -192,6 +194,7 @@ static RedArgs red(Problem& P, int op)
     ra.hist = P.hist;
     ra.hist_cap = P.hist_cap;
     ra.out = P.dout;
+    ra.cap = P.partial_cap;
     return ra;
 }
 

@@ -48,6 +48,7 @@ struct Problem
     double* hist = nullptr;
     int hist_cap = 0;
     double* partial = nullptr;
+    int64_t partial_cap = 0;   // reduction partial slots
     unsigned int* ticket = nullptr;
     double* dout = nullptr;   // OP_STORE results
     double h_out[8] = {};     // host landing slot of OP_STORE results
@@ -67,6 +68,19 @@ struct Problem
     int graph_mode = -1;
     bool graph_precond = true;
     int graph_replace = -1;
+
+    // forget captured graphs (launch shapes or operands changed)
+    void invalidate_graphs()
+    {
+        for(int k = 0; k < 2; ++k)
+        {
+            if(gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+            if(graph[k]) (void)hipGraphDestroy(graph[k]);
+            gexec[k] = nullptr;
+            graph[k] = nullptr;
+        }
+        graph_mode = -1;
+    }
 };
 
 int problem_create(Problem*& P, int nx, int ny, int nz, int nlevels, bool with_f64, bool with_mixed, hipStream_t s);
