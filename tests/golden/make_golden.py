@@ -1,0 +1,58 @@
+"""Writes the golden fixtures of tests/golden/ (run once, in a tree with the
+reference sources, after `make -C oracle oracle ref`).
+
+Every vector is computed twice -- by the oracle's restatement and by the
+reference's own _ref sources built into oracle/_ref -- with one OpenMP thread,
+and written only if both agree bit for bit.  The fixtures then pin the oracle
+wherever oracle/_ref cannot be built.
+
+  cg_history_32.npz   CG_ref, 32^3, 4 levels, b = A*ones, x0 = 0, 50 iterations:
+                      normr per iteration in natural order and in the JPL
+                      multicolour order.
+  kernels_16.npz      16x16x16 (JPL order): x ~ U[-1,1] (numpy default_rng(2024)),
+                      A x, one SymGS from x = 0 and from x, one V-cycle.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+import oracle  # noqa: E402
+
+
+def agree(a, b, what):
+    if not np.array_equal(a, b):
+        raise SystemExit(f"oracle and reference disagree on {what}")
+    return a
+
+
+def main():
+    oracle.build(ref=True)
+    oracle.set_threads(1)
+    out = {}
+    for name, order in (("natural", oracle.ORDER_NATURAL), ("jpl", oracle.ORDER_JPL)):
+        P = oracle.Problem(32, nlevels=4, order=order)
+        R = oracle.RefHierarchy(P)
+        b = P.rhs_ones()
+        h = P.cg(b, max_iter=50)["history"]
+        out[f"history_{name}"] = agree(h, R.history(b, max_iter=50), f"history {name}")
+    np.savez(os.path.join(HERE, "cg_history_32.npz"), **out)
+
+    P = oracle.Problem(16, nlevels=4, order=oracle.ORDER_JPL)
+    R = oracle.RefHierarchy(P)
+    rng = np.random.default_rng(2024)
+    x = rng.uniform(-1, 1, P.m)
+    r = rng.uniform(-1, 1, P.m)
+    k = dict(x=x, r=r)
+    k["Ax"] = agree(P.spmv(x), R.spmv(x), "spmv")
+    k["symgs_zero"] = agree(P.symgs(r, np.zeros(P.m)), R.symgs(r, np.zeros(P.m)), "symgs from 0")
+    k["symgs_x"] = agree(P.symgs(r, x.copy()), R.symgs(r, x.copy()), "symgs from x")
+    k["mg"] = agree(P.mg(r), R.mg(r), "mg")
+    np.savez(os.path.join(HERE, "kernels_16.npz"), **k)
+    print("golden fixtures written")
+
+
+if __name__ == "__main__":
+    main()

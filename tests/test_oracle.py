@@ -274,3 +274,28 @@ def test_rhs_random_is_A_times_xstar(oracle_mod):
     xs, b = P.rhs_random(1234)
     assert np.all(np.abs(xs) <= 1.0)
     assert np.allclose(P.spmv(xs), b, rtol=0, atol=1e-13)
+
+
+def _golden(name):
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", name), allow_pickle=False)
+
+
+@pytest.mark.parametrize("order,name", [(0, "natural"), (1, "jpl")])
+def test_oracle_matches_golden_cg_history(oracle_mod, order, name):
+    """Fixture written by tests/golden/make_golden.py from the reference's own
+    CG_ref (oracle/_ref) and checked equal to the oracle there."""
+    g = _golden("cg_history_32.npz")
+    P = oracle_mod.Problem(32, nlevels=4, order=order)
+    res = P.cg(P.rhs_ones(), max_iter=50)
+    assert np.array_equal(res["history"], g[f"history_{name}"])
+
+
+def test_oracle_matches_golden_kernels(oracle_mod):
+    g = _golden("kernels_16.npz")
+    P = oracle_mod.Problem(16, nlevels=4, order=1)
+    x, r = g["x"], g["r"]
+    assert np.array_equal(P.spmv(x), g["Ax"])
+    assert np.array_equal(P.symgs(r, np.zeros(P.m)), g["symgs_zero"])
+    assert np.array_equal(P.symgs(r, x.copy()), g["symgs_x"])
+    assert np.array_equal(P.mg(r), g["mg"])
