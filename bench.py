@@ -120,8 +120,8 @@ def main():
     if not args.no_tune:
         from rochpcg_amd import autotune
         t0 = time.perf_counter()
-        table = autotune.tune(G, mixed=False, reps=3, log=log)
-        table = autotune.tune(G, mixed=True, reps=3, table=table, log=log)
+        table = autotune.tune(G, mixed=False, log=log)
+        table = autotune.tune(G, mixed=True, table=table, log=log)
         log(f"autotune: {time.perf_counter() - t0:.1f} s")
         L0 = G.levels[0]
         tuned = {k: {op: v["shape"] for op, v in e.items()} for k, e in table.items()

@@ -1,0 +1,2 @@
+// OptimizeProblem.hpp -- declared in GenerateProblem.hpp (reference header name kept for drop-in includes).
+#include "GenerateProblem.hpp"

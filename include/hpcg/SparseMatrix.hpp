@@ -1,0 +1,62 @@
+// SparseMatrix.hpp -- the system matrix of the drop-in C++ layer.  Field
+// names follow ROCm/rocHPCG src/SparseMatrix.hpp:75-156 for everything the
+// optimized path touches (ELL arrays, colouring, hierarchy links).  The host
+// _ref arrays (mtxIndL, matrixValues, ...) are kept as fields but stay null:
+// this layer builds the operator on the device only.
+#ifndef HPCGX_SPARSEMATRIX_HPP
+#define HPCGX_SPARSEMATRIX_HPP
+
+#include "Geometry.hpp"
+#include "MGData.hpp"
+#include "Vector.hpp"
+
+struct SparseMatrix_STRUCT
+{
+    char* title;
+    Geometry* geom;
+    global_int_t totalNumberOfRows;
+    global_int_t totalNumberOfNonzeros;
+    local_int_t localNumberOfRows;
+    local_int_t localNumberOfColumns;
+    local_int_t localNumberOfNonzeros;
+    local_int_t numberOfNonzerosPerRow;
+    char* nonzerosInRow;
+    global_int_t** mtxIndG;
+    local_int_t** mtxIndL;
+    double** matrixValues;
+    double** matrixDiagonal;
+    mutable bool isDotProductOptimized;
+    mutable bool isSpmvOptimized;
+    mutable bool isMgOptimized;
+    mutable bool isWaxpbyOptimized;
+    mutable struct SparseMatrix_STRUCT* Ac;
+    mutable MGData* mgData;
+    void* optimizationData;   // the solver level behind this matrix (set by OptimizeProblem)
+
+    // ELL storage (column-major: entry p of row r at p*localNumberOfRows + r,
+    // 64-bit offsets), multicolour row order
+    local_int_t ell_width;
+    local_int_t* ell_col_ind;
+    double* ell_val;
+    local_int_t* diag_idx;
+    double* inv_diag;
+
+    // SymGS colouring
+    int nblocks;
+    int ublocks;
+    local_int_t* sizes;
+    local_int_t* offsets;
+    local_int_t* perm;   // natural row -> multicolour row
+};
+typedef struct SparseMatrix_STRUCT SparseMatrix;
+
+void InitializeSparseMatrix(SparseMatrix& A, Geometry* geom);
+void DeleteMatrix(SparseMatrix& A);
+
+// SparseMatrix.hpp:220-255 of the reference (device diagonal, multicolour order)
+void HIPCopyMatrixDiagonal(const SparseMatrix& A, Vector& diagonal);
+void HIPReplaceMatrixDiagonal(SparseMatrix& A, const Vector& diagonal);
+inline void CopyMatrixDiagonal(SparseMatrix& A, Vector& diagonal) { HIPCopyMatrixDiagonal(A, diagonal); }
+inline void ReplaceMatrixDiagonal(SparseMatrix& A, Vector& diagonal) { HIPReplaceMatrixDiagonal(A, diagonal); }
+
+#endif

@@ -66,10 +66,23 @@ def apply(prob, table):
     return n
 
 
-def tune(prob, mixed=False, reps=5, levels=None, table=None, log=None):
-    """Time every shape of every tunable operator; apply and return the table."""
+def _prime(prob, mixed):
+    """Operands as in a solve: r = b - A*0 from the CG preamble, and every
+    level's r / x filled by one V-cycle."""
+    from . import CG_F64, CG_MIXED, OP_MG
+    prob.cg_begin(mode=CG_MIXED if mixed else CG_F64, max_iter=1, tol=0.0)
+    prob.cg_end()
+    prob.time_operator(OP_MG, level=0, mixed=mixed, reps=1)
+
+
+def tune(prob, mixed=False, reps=3, rounds=3, levels=None, table=None, log=None):
+    """Time every shape of every tunable operator -- candidates interleaved
+    over `rounds` rounds, median per candidate -- then apply the fastest and
+    return the table."""
+    import statistics
     table = {} if table is None else table
     last = prob.nlevels - 1
+    _prime(prob, mixed)
     for lv in (range(prob.nlevels) if levels is None else levels):
         L = prob.levels[lv]
         if not (L.stores & (STORE_MIXED if mixed else STORE_F64)):
@@ -79,10 +92,12 @@ def tune(prob, mixed=False, reps=5, levels=None, table=None, log=None):
             if op == SH_RESTRICT and lv == last:
                 continue
             cands = SHAPES if op != SH_RESTRICT else [(b, 1) for b in (64, 128, 256)]
-            timings = {}
-            for block, rows in cands:
-                prob.set_shape(block, rows, level=lv, op=op, mixed=mixed)
-                timings[(block, rows)] = prob.time_operator(top, level=lv, mixed=mixed, reps=reps)
+            samples = {c: [] for c in cands}
+            for _ in range(rounds):
+                for block, rows in cands:
+                    prob.set_shape(block, rows, level=lv, op=op, mixed=mixed)
+                    samples[(block, rows)].append(prob.time_operator(top, level=lv, mixed=mixed, reps=reps))
+            timings = {c: statistics.median(v) for c, v in samples.items()}
             (block, rows), best = min(timings.items(), key=lambda kv: kv[1])
             prob.set_shape(block, rows, level=lv, op=op, mixed=mixed)
             default = timings.get((256, 1))

@@ -1,1 +1,513 @@
-// placeholder
+// hpcg_api.cpp -- the reference-named C++ layer (include/hpcg/*.hpp) over the
+// solver: SparseMatrix / Vector / MGData / CGData and GenerateProblem,
+// GenerateCoarseProblem, OptimizeProblem, ComputeSPMV, ComputeSYMGS(ZeroGuess),
+// ComputeMG, ComputeDotProduct, ComputeWAXPBY, ComputeFusedWAXPBYDot,
+// ComputeRestriction, ComputeFusedSpMVRestriction, ComputeProlongation, CG.
+//
+// Like the reference (utils.hpp:37-41: stream_interior, workspace), the layer
+// keeps one process-wide stream and reduction scratch; every call is
+// asynchronous on that stream except the dot products, which return host
+// values.  Errors return non-zero (the reference exits in HIP_CHECK).
+#include "../../include/hpcg/hpcg.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ops.hpp"
+#include "setup.hpp"
+#include "solver.hpp"
+
+using namespace hpcgx;
+
+namespace
+{
+
+struct Context
+{
+    hipStream_t s = nullptr;
+    double* partial = nullptr;
+    unsigned* ticket = nullptr;
+    double* dout = nullptr;
+    double h_out = 0.0;
+    bool ok = false;
+};
+
+Context& ctx()
+{
+    static Context c;
+    if(!c.ok)
+    {
+        if(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking) != hipSuccess ||
+           hipMalloc(&c.partial, sizeof(double) * RED_BLOCKS) != hipSuccess ||
+           hipMalloc(&c.ticket, 4 * sizeof(unsigned)) != hipSuccess ||
+           hipMalloc(&c.dout, 8 * sizeof(double)) != hipSuccess ||
+           hipMemsetAsync(c.ticket, 0, 4 * sizeof(unsigned), c.s) != hipSuccess)
+        {
+            fprintf(stderr, "rochpcg_amd: cannot initialise the HIP context\n");
+            return c;
+        }
+        c.ok = true;
+    }
+    return c;
+}
+
+struct ApiLevel
+{
+    Problem* P;
+    int level;
+};
+
+bool level_of(const SparseMatrix& A, Problem*& P, int& l)
+{
+    const ApiLevel* al = (const ApiLevel*)A.optimizationData;
+    if(!al || !al->P)
+    {
+        set_error("matrix is not optimized (call OptimizeProblem first)");
+        return false;
+    }
+    P = al->P;
+    l = al->level;
+    return true;
+}
+
+RedArgs store_red()
+{
+    RedArgs ra;
+    ra.partial = ctx().partial;
+    ra.ticket = ctx().ticket;
+    ra.op = OP_STORE;
+    ra.out = ctx().dout;
+    ra.cap = RED_BLOCKS;
+    return ra;
+}
+
+int fetch_result(double& result)
+{
+    Context& c = ctx();
+    HPCGX_HIP(hipMemcpyAsync(&c.h_out, c.dout, sizeof(double), hipMemcpyDeviceToHost, c.s));
+    HPCGX_HIP(hipStreamSynchronize(c.s));
+    result = c.h_out;
+    return 0;
+}
+
+void* dalloc(size_t bytes)
+{
+    void* p = nullptr;
+    if(hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) return nullptr;
+    return p;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// geometry, vectors, structs
+
+void GenerateGeometry(int size, int rank, int numThreads, int pz, local_int_t zl, local_int_t zu, local_int_t nx,
+                      local_int_t ny, local_int_t nz, int npx, int npy, int npz, Geometry* geom)
+{
+    std::memset(geom, 0, sizeof(*geom));
+    geom->size = size;
+    geom->rank = rank;
+    geom->numThreads = numThreads;
+    geom->nx = nx;
+    geom->ny = ny;
+    geom->nz = nz;
+    geom->npx = npx > 0 ? npx : 1;
+    geom->npy = npy > 0 ? npy : 1;
+    geom->npz = npz > 0 ? npz : 1;
+    geom->pz = pz;
+    geom->npartz = 1;
+    geom->partz_ids = new int[1]{geom->npz};
+    geom->partz_nz = new local_int_t[1]{nz};
+    (void)zl;
+    (void)zu;
+    geom->gnx = (global_int_t)geom->npx * nx;
+    geom->gny = (global_int_t)geom->npy * ny;
+    geom->gnz = (global_int_t)geom->npz * nz;
+}
+
+void DeleteGeometry(Geometry& geom)
+{
+    delete[] geom.partz_ids;
+    delete[] geom.partz_nz;
+    geom.partz_ids = nullptr;
+    geom.partz_nz = nullptr;
+}
+
+void HIPInitializeVector(Vector& v, local_int_t localLength)
+{
+    v.localLength = localLength;
+    v.optimizationData = nullptr;
+    v.d_values = (double*)dalloc(sizeof(double) * (size_t)localLength);
+}
+
+void HIPZeroVector(Vector& v) { (void)launch_fill(v.localLength, v.d_values, true, 0.0, ctx().s, nullptr); }
+
+void HIPFillRandomVector(Vector& v)
+{
+    std::vector<double> h((size_t)v.localLength);
+    for(auto& e : h) e = rand() / (double)(RAND_MAX) + 1.0;
+    (void)hipStreamSynchronize(ctx().s);
+    (void)hipMemcpy(v.d_values, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice);
+}
+
+void HIPCopyVector(const Vector& v, Vector& w)
+{
+    (void)launch_copy(v.localLength, v.d_values, true, w.d_values, true, ctx().s, nullptr);
+}
+
+void HIPDeleteVector(Vector& v)
+{
+    if(v.d_values)
+    {
+        (void)hipStreamSynchronize(ctx().s);
+        (void)hipFree(v.d_values);
+    }
+    v.d_values = nullptr;
+    v.localLength = 0;
+}
+
+void InitializeMGData(local_int_t* d_f2cOperator, local_int_t* d_c2fOperator, Vector* rc, Vector* xc, Vector* Axf,
+                      MGData& data)
+{
+    data.numberOfPresmootherSteps = 1;
+    data.numberOfPostsmootherSteps = 1;
+    data.f2cOperator = nullptr;
+    data.d_f2cOperator = d_f2cOperator;
+    data.d_c2fOperator = d_c2fOperator;
+    data.rc = rc;
+    data.xc = xc;
+    data.Axf = Axf;
+    data.optimizationData = nullptr;
+    data.owns_vectors = true;
+}
+
+void DeleteMGData(MGData& data)
+{
+    (void)hipStreamSynchronize(ctx().s);
+    delete[] data.f2cOperator;
+    if(data.owns_vectors)
+    {
+        HIPDeleteVector(*data.rc);
+        HIPDeleteVector(*data.xc);
+    }
+    HIPDeleteVector(*data.Axf);
+    delete data.rc;
+    delete data.xc;
+    delete data.Axf;
+    if(data.d_f2cOperator) (void)hipFree(data.d_f2cOperator);
+    if(data.d_c2fOperator) (void)hipFree(data.d_c2fOperator);
+    data.rc = data.xc = data.Axf = nullptr;
+    data.d_f2cOperator = data.d_c2fOperator = nullptr;
+}
+
+void InitializeSparseMatrix(SparseMatrix& A, Geometry* geom)
+{
+    std::memset(&A, 0, sizeof(A));
+    A.geom = geom;
+    A.isDotProductOptimized = true;
+    A.isSpmvOptimized = true;
+    A.isMgOptimized = true;
+    A.isWaxpbyOptimized = true;
+}
+
+void DeleteMatrix(SparseMatrix& A)
+{
+    if(A.Ac)
+    {
+        DeleteMatrix(*A.Ac);
+        delete A.Ac;
+        A.Ac = nullptr;
+    }
+    if(A.mgData)
+    {
+        DeleteMGData(*A.mgData);
+        delete A.mgData;
+        A.mgData = nullptr;
+    }
+    ApiLevel* al = (ApiLevel*)A.optimizationData;
+    if(al)
+    {
+        if(al->level == 0) problem_destroy(al->P);
+        delete al;
+        A.optimizationData = nullptr;
+    }
+    delete[] A.sizes;
+    delete[] A.offsets;
+    A.sizes = A.offsets = nullptr;
+    if(A.geom)
+    {
+        DeleteGeometry(*A.geom);
+        delete A.geom;
+        A.geom = nullptr;
+    }
+}
+
+void HIPInitializeSparseCGData(SparseMatrix& A, CGData& data)
+{
+    HIPInitializeVector(data.r, A.localNumberOfRows);
+    HIPInitializeVector(data.z, A.localNumberOfColumns);
+    HIPInitializeVector(data.p, A.localNumberOfColumns);
+    HIPInitializeVector(data.Ap, A.localNumberOfRows);
+}
+
+void HIPDeleteCGData(CGData& data)
+{
+    HIPDeleteVector(data.r);
+    HIPDeleteVector(data.z);
+    HIPDeleteVector(data.p);
+    HIPDeleteVector(data.Ap);
+}
+
+// ---------------------------------------------------------------------------
+// problem construction
+
+static void size_from_geometry(SparseMatrix& A)
+{
+    const Geometry& g = *A.geom;
+    const global_int_t m = (global_int_t)g.nx * g.ny * g.nz;
+    A.totalNumberOfRows = (global_int_t)g.gnx * g.gny * g.gnz;
+    A.totalNumberOfNonzeros = (global_int_t)(3 * g.gnx - 2) * (3 * g.gny - 2) * (3 * g.gnz - 2);
+    A.localNumberOfRows = (local_int_t)m;
+    A.localNumberOfColumns = (local_int_t)m;
+    A.localNumberOfNonzeros = (local_int_t)((3 * g.nx - 2) * (3 * g.ny - 2) * (3 * g.nz - 2));
+    A.numberOfNonzerosPerRow = 27;
+}
+
+void GenerateProblem(SparseMatrix& A, Vector* b, Vector* x, Vector* xexact)
+{
+    size_from_geometry(A);
+    const Geometry& g = *A.geom;
+    const local_int_t m = A.localNumberOfRows;
+    if(b) HIPInitializeVector(*b, m);
+    if(x) HIPInitializeVector(*x, m);
+    if(xexact) HIPInitializeVector(*xexact, m);
+    (void)natural_rhs(g.nx, g.ny, g.nz, b ? b->d_values : nullptr, xexact ? xexact->d_values : nullptr,
+                      x ? x->d_values : nullptr, ctx().s);
+}
+
+void SetupHalo(SparseMatrix& A) { (void)A; }
+
+void GenerateCoarseProblem(const SparseMatrix& Af)
+{
+    const Geometry& gf = *Af.geom;
+    Geometry* gc = new Geometry;
+    GenerateGeometry(gf.size, gf.rank, gf.numThreads, gf.pz, 0, 0, gf.nx / 2, gf.ny / 2, gf.nz / 2, gf.npx, gf.npy,
+                     gf.npz, gc);
+    SparseMatrix* Ac = new SparseMatrix;
+    InitializeSparseMatrix(*Ac, gc);
+    GenerateProblem(*Ac, nullptr, nullptr, nullptr);
+    const local_int_t mc = Ac->localNumberOfRows;
+    local_int_t* f2c = (local_int_t*)dalloc(sizeof(local_int_t) * (size_t)mc);
+    local_int_t* c2f = (local_int_t*)dalloc(sizeof(local_int_t) * (size_t)Af.localNumberOfRows);
+    (void)natural_transfer(gf.nx, gf.ny, gf.nz, f2c, c2f, ctx().s);
+    Vector* rc = new Vector;
+    Vector* xc = new Vector;
+    Vector* Axf = new Vector;
+    HIPInitializeVector(*rc, mc);
+    HIPInitializeVector(*xc, mc);
+    HIPInitializeVector(*Axf, Af.localNumberOfColumns);
+    MGData* mg = new MGData;
+    InitializeMGData(f2c, c2f, rc, xc, Axf, *mg);
+    Af.Ac = Ac;
+    Af.mgData = mg;
+}
+
+int OptimizeProblem(SparseMatrix& A, CGData& data, Vector& b, Vector& x, Vector& xexact)
+{
+    if(!A.geom || A.geom->size != 1 || A.geom->npx * A.geom->npy * A.geom->npz != 1)
+    {
+        fprintf(stderr, "OptimizeProblem: this library runs one rank on one GPU\n");
+        return 1;
+    }
+    int nlevels = 1;
+    for(const SparseMatrix* M = &A; M->Ac; M = M->Ac) ++nlevels;
+    if(nlevels > MAX_LEVELS)
+    {
+        fprintf(stderr, "OptimizeProblem: too many levels\n");
+        return 1;
+    }
+    Context& c = ctx();
+    if(!c.ok) return 1;
+    Problem* P = nullptr;
+    const Geometry& g = *A.geom;
+    if(problem_create(P, g.nx, g.ny, g.nz, nlevels, true, false, c.s) != 0)
+    {
+        fprintf(stderr, "OptimizeProblem: %s\n", last_error());
+        return 1;
+    }
+    SparseMatrix* M = &A;
+    for(int l = 0; l < nlevels; ++l, M = M->Ac)
+    {
+        const Level& L = P->L[l];
+        M->optimizationData = new ApiLevel{P, l};
+        M->ell_width = W;
+        M->ell_col_ind = L.ell_col;
+        M->ell_val = L.ell_val;
+        M->diag_idx = L.diag_idx;
+        M->inv_diag = L.inv_diag;
+        M->perm = L.perm;
+        M->nblocks = L.ncolors;
+        M->ublocks = L.ncolors - 2;   // the backward sweep starts at colour nblocks-2 (MultiColoring.cpp:350-352)
+        delete[] M->sizes;
+        delete[] M->offsets;
+        M->sizes = new local_int_t[L.ncolors];
+        M->offsets = new local_int_t[L.ncolors + 1];
+        for(int k = 0; k < L.ncolors; ++k)
+        {
+            M->sizes[k] = L.sizes[k];
+            M->offsets[k] = L.offsets[k];
+        }
+        M->offsets[L.ncolors] = L.offsets[L.ncolors];
+        if(M->mgData)
+        {
+            // the solver's coarse buffers replace the ones GenerateCoarseProblem made
+            MGData& mg = *M->mgData;
+            if(mg.owns_vectors)
+            {
+                HIPDeleteVector(*mg.rc);
+                HIPDeleteVector(*mg.xc);
+                mg.rc->d_values = (double*)P->v64[l + 1].r;
+                mg.xc->d_values = (double*)P->v64[l + 1].x;
+                mg.rc->localLength = mg.xc->localLength = (local_int_t)P->L[l + 1].m;
+                mg.owns_vectors = false;
+            }
+        }
+    }
+    // PermuteVector (Permute.cpp:220-246): b, x, xexact into multicolour order
+    Vector* vs[] = {&b, &x, &xexact};
+    for(Vector* v : vs)
+    {
+        double* out = (double*)dalloc(sizeof(double) * (size_t)A.localNumberOfRows);
+        if(!out || permute_vector(A.localNumberOfRows, A.perm, v->d_values, out, c.s)) return 1;
+        (void)hipStreamSynchronize(c.s);
+        (void)hipFree(v->d_values);
+        v->d_values = out;
+    }
+    HIPInitializeSparseCGData(A, data);
+    if(adopt_cg_vectors(*P, data.r.d_values, data.z.d_values, data.p.d_values, data.Ap.d_values)) return 1;
+    return hipStreamSynchronize(c.s) == hipSuccess ? 0 : 1;
+}
+
+// ---------------------------------------------------------------------------
+// operators
+
+int ComputeSPMV(const SparseMatrix& A, Vector& x, Vector& y)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l)) return 1;
+    if(A.mgData && &y == A.mgData->Axf) return launch_spmv_injected(P->L[l], x.d_values, y.d_values, P->s) ? 1 : 0;
+    return launch_spmv_f64x(P->L[l], PREC_F64, x.d_values, y.d_values, nullptr, P->s, nullptr) ? 1 : 0;
+}
+
+int ComputeSYMGS(const SparseMatrix& A, const Vector& r, Vector& x)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l)) return 1;
+    return launch_symgs(P->L[l], PREC_F64, r.d_values, true, x.d_values, P->v64[l].t, P->s, nullptr) ? 1 : 0;
+}
+
+int ComputeSYMGSZeroGuess(const SparseMatrix& A, const Vector& r, Vector& x)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l)) return 1;
+    return launch_symgs_zero(P->L[l], PREC_F64, r.d_values, true, x.d_values, P->s, nullptr) ? 1 : 0;
+}
+
+int ComputeMG(const SparseMatrix& A, const Vector& r, Vector& x)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l)) return 1;
+    return mg(*P, l, PREC_F64, r.d_values, true, x.d_values, nullptr) ? 1 : 0;
+}
+
+int ComputeRestriction(const SparseMatrix& A, const Vector& rf)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l) || !A.mgData) return 1;
+    return launch_restrict_diff(P->L[l], rf.d_values, A.mgData->Axf->d_values, A.mgData->rc->d_values, P->s) ? 1 : 0;
+}
+
+int ComputeFusedSpMVRestriction(const SparseMatrix& A, const Vector& rf, Vector& xf)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l) || !A.mgData) return 1;
+    return launch_restrict(P->L[l], PREC_F64, rf.d_values, true, xf.d_values, A.mgData->rc->d_values, P->s, nullptr)
+               ? 1
+               : 0;
+}
+
+int ComputeProlongation(const SparseMatrix& Af, Vector& xf)
+{
+    Problem* P;
+    int l;
+    if(!level_of(Af, P, l) || !Af.mgData) return 1;
+    return launch_prolong(P->L[l], PREC_F64, Af.mgData->xc->d_values, xf.d_values, P->s, nullptr) ? 1 : 0;
+}
+
+int ComputeDotProduct(const local_int_t n, const Vector& x, const Vector& y, double& result, double& time_allreduce,
+                      bool& isOptimized)
+{
+    (void)isOptimized;
+    time_allreduce += 0.0;
+    if(launch_dot(n, x.d_values, true, y.d_values, true, store_red(), ctx().s, nullptr)) return 1;
+    return fetch_result(result) ? 1 : 0;
+}
+
+int ComputeWAXPBY(const local_int_t n, const double alpha, const Vector& x, const double beta, const Vector& y,
+                  Vector& w, bool& isOptimized)
+{
+    (void)isOptimized;
+    return launch_waxpby(n, alpha, x.d_values, beta, y.d_values, w.d_values, ctx().s, nullptr) ? 1 : 0;
+}
+
+int ComputeFusedWAXPBYDot(const local_int_t n, const double alpha, const Vector& x, Vector& y, double& result,
+                          double& time_allreduce)
+{
+    time_allreduce += 0.0;
+    if(launch_axpy_dot(n, alpha, x.d_values, y.d_values, store_red(), ctx().s, nullptr)) return 1;
+    return fetch_result(result) ? 1 : 0;
+}
+
+void HIPCopyMatrixDiagonal(const SparseMatrix& A, Vector& diagonal)
+{
+    Problem* P;
+    int l;
+    if(level_of(A, P, l)) (void)launch_get_diag(P->L[l], diagonal.d_values, P->s);
+}
+
+void HIPReplaceMatrixDiagonal(SparseMatrix& A, const Vector& diagonal)
+{
+    Problem* P;
+    int l;
+    if(level_of(A, P, l)) (void)launch_set_diag(P->L[l], diagonal.d_values, P->s);
+}
+
+int CG(const SparseMatrix& A, CGData& data, const Vector& b, Vector& x, const int max_iter, const double tolerance,
+       int& niters, double& normr, double& normr0, double* times, bool doPreconditioning, bool verbose)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l) || l != 0) return 1;
+    (void)data;   // the solver runs on these very buffers (adopted by OptimizeProblem)
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<double> hist(verbose ? (size_t)max_iter + 1 : 0);
+    const int rc = cg_solve(*P, CG_F64, doPreconditioning, b.d_values, x.d_values, max_iter, tolerance, true, &niters,
+                            &normr, &normr0, verbose ? hist.data() : nullptr, verbose ? max_iter + 1 : 0);
+    if(rc) return 1;
+    if(times) times[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if(verbose && A.geom->rank == 0)
+    {
+        printf("HIP Initial Residual = %le\n", normr0);
+        for(int k = 1; k <= niters; ++k) printf("HIP Iteration = %d   Scaled Residual = %le\n", k, hist[k] / normr0);
+    }
+    return 0;
+}

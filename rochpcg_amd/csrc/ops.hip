@@ -516,6 +516,36 @@ __global__ __launch_bounds__(BS) void k_prolong(int64_t mc, const int32_t* __res
     xf[f] = xf[f] + (XT)xc[f2c_coarse[k]];
 }
 
+// Reference-mode pieces (ComputeSPMV.cpp:87-128 on MGData::Axf and
+// ComputeRestriction.cpp:59-79): A x at the injected rows only, and
+// rc = rf - Axf at those rows.
+__global__ __launch_bounds__(256) void k_spmv_rows(int64_t m, int64_t mc, const int32_t* __restrict__ col,
+                                                   const double* __restrict__ val, const int32_t* __restrict__ rows,
+                                                   const double* __restrict__ x, double* __restrict__ y)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if(k >= mc) return;
+    const int64_t row = rows[k];
+    double s = 0.0;
+    for(int p = 0; p < W; ++p)
+    {
+        const int32_t c = col[(int64_t)p * m + row];
+        if(c >= 0) s = fmaT(val[(int64_t)p * m + row], x[c], s);
+    }
+    y[row] = s;
+}
+
+__global__ __launch_bounds__(256) void k_restrict_diff(int64_t mc, const int32_t* __restrict__ f2c_fine,
+                                                       const int32_t* __restrict__ f2c_coarse,
+                                                       const double* __restrict__ rf, const double* __restrict__ Axf,
+                                                       double* __restrict__ rc)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if(k >= mc) return;
+    const int64_t f = f2c_fine[k];
+    rc[f2c_coarse[k]] = rf[f] - Axf[f];
+}
+
 // ---------------------------------------------------------------------------
 // CG vector kernels (f64 vectors; z may be f32 in the mixed mode).  Chunked:
 // block L (XCD-contiguous) handles elements [L*chunk, (L+1)*chunk).
@@ -883,6 +913,27 @@ int launch_prolong(const Level& F, Prec prec, const void* xc, void* x, hipStream
     else
         hipLaunchKernelGGL((k_prolong<float, float, BS>), dim3(nblk(F.mc, BS)), dim3(BS), 0, s, F.mc, F.f2c_fine,
                            F.f2c_coarse, (const float*)xc, (float*)x, gate);
+    HPCGX_KCHECK();
+    return 0;
+}
+
+int launch_spmv_injected(const Level& F, const double* x, double* y, hipStream_t s)
+{
+    if(!F.ell_val || F.mc == 0)
+    {
+        set_error("injected-row SpMV needs an f64 level with a coarser level");
+        return -2;
+    }
+    hipLaunchKernelGGL(k_spmv_rows, dim3(nblk(F.mc, 256)), dim3(256), 0, s, F.m, F.mc, F.ell_col, F.ell_val, F.f2c_fine,
+                       x, y);
+    HPCGX_KCHECK();
+    return 0;
+}
+
+int launch_restrict_diff(const Level& F, const double* rf, const double* Axf, double* rc, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_restrict_diff, dim3(nblk(F.mc, 256)), dim3(256), 0, s, F.mc, F.f2c_fine, F.f2c_coarse, rf,
+                       Axf, rc);
     HPCGX_KCHECK();
     return 0;
 }

@@ -43,6 +43,8 @@ int launch_symgs(const Level& L, Prec prec, const void* r, bool r_f64, void* x, 
 int launch_restrict(const Level& F, Prec prec, const void* r, bool r_f64, const void* x, void* rc, hipStream_t s,
                     const int* gate);
 int launch_prolong(const Level& F, Prec prec, const void* xc, void* x, hipStream_t s, const int* gate);
+int launch_spmv_injected(const Level& F, const double* x, double* y, hipStream_t s);
+int launch_restrict_diff(const Level& F, const double* rf, const double* Axf, double* rc, hipStream_t s);
 int launch_dot(int64_t n, const void* x, bool x_f64, const void* y, bool y_f64, const RedArgs& ra, hipStream_t s,
                const int* gate);
 int launch_waxpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* w, hipStream_t s,

@@ -244,7 +244,7 @@ __device__ __forceinline__ double uniform01(uint64_t seed, int64_t i)
     return (double)(z >> 11) * (1.0 / 9007199254740992.0);
 }
 
-// Right-hand sides in multicolour order.  kind 0: xexact = 1, b = 27 - nnz_row
+// Right-hand sides in multicolour order (natural order when perm is null).  kind 0: xexact = 1, b = 27 - nnz_row
 // (GenerateProblem.cpp:300); kind 1: xexact = 2u-1, b = A xexact summed in
 // natural stencil order with fma.
 __global__ __launch_bounds__(256) void k_rhs(Grid g, int64_t m, int kind, uint64_t seed,
@@ -287,14 +287,64 @@ __global__ __launch_bounds__(256) void k_rhs(Grid g, int64_t m, int kind, uint64
         bv = sum;
         xv = 2.0 * uniform01(seed, i) - 1.0;
     }
-    const int64_t p = perm[i];
+    const int64_t p = perm ? perm[i] : i;
     if(b) b[p] = bv;
     if(xexact) xexact[p] = xv;
     if(x0) x0[p] = 0.0;
 }
 
+// Natural-order injection maps: f2c[k] = fine row of coarse row k, c2f[f] =
+// coarse row of fine row f (the fill kernel sets -1 first).
+__global__ __launch_bounds__(256) void k_f2c_natural(Grid gc, int nxf, int nyf, int64_t mc,
+                                                     int32_t* __restrict__ f2c, int32_t* __restrict__ c2f)
+{
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if(k >= mc) return;
+    int kx, ky, kz;
+    gc.coords(k, kx, ky, kz);
+    const int64_t f = (int64_t)(2 * kz) * nxf * nyf + (int64_t)(2 * ky) * nxf + 2 * kx;
+    f2c[k] = (int32_t)f;
+    if(c2f) c2f[f] = (int32_t)k;
+}
+
+// out[perm[i]] = in[i]
+__global__ __launch_bounds__(256) void k_permute(int64_t m, const int32_t* __restrict__ perm, const double* __restrict__ in,
+                                                 double* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if(i < m) out[perm[i]] = in[i];
+}
+
 // ---------------------------------------------------------------------------
 static inline unsigned grid_of(int64_t n, int bs) { return (unsigned)ceil_div(n, bs); }
+
+int natural_transfer(int nxf, int nyf, int nzf, int32_t* f2c, int32_t* c2f, hipStream_t s)
+{
+    Grid gc{nxf / 2, nyf / 2, nzf / 2};
+    const int64_t mc = (int64_t)gc.nx * gc.ny * gc.nz;
+    const int64_t mf = (int64_t)nxf * nyf * nzf;
+    if(c2f) hipLaunchKernelGGL(k_fill_i32, dim3(grid_of(mf, 256)), dim3(256), 0, s, mf, c2f, -1);
+    hipLaunchKernelGGL(k_f2c_natural, dim3(grid_of(mc, 256)), dim3(256), 0, s, gc, nxf, nyf, mc, f2c, c2f);
+    HPCGX_HIP(hipGetLastError());
+    return 0;
+}
+
+int permute_vector(int64_t m, const int32_t* perm, const double* in, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_permute, dim3(grid_of(m, 256)), dim3(256), 0, s, m, perm, in, out);
+    HPCGX_HIP(hipGetLastError());
+    return 0;
+}
+
+int natural_rhs(int nx, int ny, int nz, double* b, double* xexact, double* x0, hipStream_t s)
+{
+    Grid g{nx, ny, nz};
+    const int64_t m = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_rhs, dim3(grid_of(m, 256)), dim3(256), 0, s, g, m, 0, (uint64_t)0, (const int32_t*)nullptr, b,
+                       xexact, x0);
+    HPCGX_HIP(hipGetLastError());
+    return 0;
+}
 
 int color_level(Level& L, hipStream_t s)
 {

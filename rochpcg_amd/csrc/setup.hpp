@@ -14,4 +14,8 @@ int build_transfer(Level& F, Level& C, hipStream_t s);
 // b / xexact / x0 in multicolour order (kind 0: xexact = 1; kind 1: seeded uniform [-1,1]).
 int make_rhs(const Level& L, int kind, uint64_t seed, double* b, double* xexact, double* x0, hipStream_t s);
 void free_level(Level& L);
+// natural-order helpers of the drop-in C++ layer
+int natural_transfer(int nxf, int nyf, int nzf, int32_t* f2c, int32_t* c2f, hipStream_t s);
+int permute_vector(int64_t m, const int32_t* perm, const double* in, double* out, hipStream_t s);
+int natural_rhs(int nx, int ny, int nz, double* b, double* xexact, double* x0, hipStream_t s);
 }  // namespace hpcgx

@@ -147,10 +147,13 @@ void problem_destroy(Problem* P)
     f(P->b);
     f(P->x);
     f(P->xexact);
-    f(P->r);
-    f(P->p);
-    f(P->Ap);
-    f(P->z64);
+    if(!P->ext_cg)
+    {
+        f(P->r);
+        f(P->p);
+        f(P->Ap);
+        f(P->z64);
+    }
     f(P->z32);
     f(P->tmp);
     f(P->sc);
@@ -160,6 +163,30 @@ void problem_destroy(Problem* P)
     f(P->dout);
     if(P->own_stream && P->s) (void)hipStreamDestroy(P->s);
     delete P;
+}
+
+int adopt_cg_vectors(Problem& P, double* r, double* z, double* p, double* Ap)
+{
+    if(!r || !z || !p || !Ap)
+    {
+        set_error("null CG vector");
+        return -2;
+    }
+    HPCGX_HIP(hipStreamSynchronize(P.s));
+    if(!P.ext_cg)
+    {
+        (void)hipFree(P.r);
+        (void)hipFree(P.z64);
+        (void)hipFree(P.p);
+        (void)hipFree(P.Ap);
+    }
+    P.r = r;
+    P.z64 = z;
+    P.p = p;
+    P.Ap = Ap;
+    P.ext_cg = true;
+    P.invalidate_graphs();
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

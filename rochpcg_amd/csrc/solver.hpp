@@ -51,6 +51,7 @@ struct Problem
     int64_t partial_cap = 0;   // reduction partial slots
     unsigned int* ticket = nullptr;
     double* dout = nullptr;   // OP_STORE results
+    bool ext_cg = false;      // r/z64/p/Ap belong to the caller (drop-in CGData)
     double h_out[8] = {};     // host landing slot of OP_STORE results
     CGScalars h_sc = {};      // host image of the scalars uploaded by cg_begin
 
@@ -85,6 +86,8 @@ struct Problem
 
 int problem_create(Problem*& P, int nx, int ny, int nz, int nlevels, bool with_f64, bool with_mixed, hipStream_t s);
 void problem_destroy(Problem* P);
+// use caller-owned CG vectors (CGData of the drop-in layer) instead of the problem's own
+int adopt_cg_vectors(Problem& P, double* r, double* z, double* p, double* Ap);
 
 // V-cycle on `level` (r is f64 at level 0 of both modes; x is f64 or f32)
 int mg(Problem& P, int level, Prec prec, const void* r, bool r_f64, void* x, const int* gate);

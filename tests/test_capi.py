@@ -38,3 +38,14 @@ def test_hpcg_flop_model():
     f = vv.hpcg_flops_per_iteration((320, 320, 320))
     assert abs(f / 1e9 - 12.177) < 0.001
     assert vv.check_niter(32) == 53 and vv.check_niter(288) == 51 and vv.check_niter(320) == 50
+
+
+def test_drop_in_headers_compile_standalone(tmp_path):
+    """include/hpcg needs no HIP headers: user code builds with plain g++."""
+    import subprocess
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "hpcg.hpp"\nint main(){ SparseMatrix A; InitializeSparseMatrix(A, nullptr);'
+                   ' return A.ell_width; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include", "hpcg"), str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
