@@ -87,6 +87,57 @@ def cpu_baseline(dims, iters, threads):
                 s_per_iter=round(el / max(res["niters"], 1), 3))
 
 
+def batched_leg(hip, count, log):
+    """64 independent problems, nx/ny/nz each from {16,...,96} (splitmix64 seed 7),
+    problem k with x* uniform in [-1,1] from seed 7+k; solved to 1e-9 together
+    (one launch per operator/level/colour for all problems, graph-replayed)."""
+    import numpy as np
+    from rochpcg_amd import vv
+    dims = vv.batched_dims(count, seed=7)
+    t0 = time.perf_counter()
+    B = hip.Batch(dims, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
+    setup = time.perf_counter() - t0
+    for k, P in enumerate(B.problems):
+        P.make_rhs(1, 7 + k)
+    flops = np.array([vv.hpcg_flops_per_iteration(d) for d in dims])
+    # block size of the batched kernels: pick the fastest of a short FP64 run
+    best = None
+    for block in (64, 128, 256):
+        for lv in range(4):
+            B.set_block(lv, block)
+        for P in B.problems:
+            P.vec["x"].upload(np.zeros(P.m))
+        B.cg(max_iter=3)   # graph capture
+        for P in B.problems:
+            P.vec["x"].upload(np.zeros(P.m))
+        t = time.perf_counter()
+        B.cg(max_iter=10)
+        el = time.perf_counter() - t
+        if best is None or el < best[0]:
+            best = (el, block)
+    for lv in range(4):
+        B.set_block(lv, best[1])
+    res = {"problems": count, "rows_total": int(sum(a * b * c for a, b, c in dims)),
+           "setup_s": round(setup, 3), "block": best[1]}
+    for name, mode in (("f64", hip.CG_F64), ("mixed", hip.CG_MIXED)):
+        for P in B.problems:
+            P.vec["x"].upload(np.zeros(P.m))
+        B.cg(max_iter=2, mode=mode)   # capture this mode's graph
+        for P in B.problems:
+            P.vec["x"].upload(np.zeros(P.m))
+        t = time.perf_counter()
+        r = B.cg(max_iter=500, tol=1e-9, mode=mode)
+        el = time.perf_counter() - t
+        worst = float(np.max(r["normr"] / r["normr0"]))
+        res[name] = {"seconds": round(el, 4), "iterations_max": int(r["niters"].max()),
+                     "iterations_min": int(r["niters"].min()), "worst_rel_residual": float(f"{worst:.3e}"),
+                     "gflops": round(float((flops * r["niters"]).sum()) / el / 1e9, 2)}
+        log(f"batched {name}: {res[name]}")
+    res["speedup_time_to_solution"] = round(res["f64"]["seconds"] / res["mixed"]["seconds"], 3)
+    B.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +151,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of a hipGraph")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (256 threads, 1 row)")
+    ap.add_argument("--seed", type=int, default=1234, help="splitmix64 seed of x* in the solve-to-1e-9 leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-problems leg")
+    ap.add_argument("--batch-count", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs (--gpus > 1) "
@@ -193,8 +247,8 @@ def main():
     }
 
     if not args.no_solve:
-        # solve to 1e-9 from x0 = 0 with x* uniform in [-1,1] (splitmix64, seed 1234)
-        G.make_rhs(1, 1234)
+        # solve to 1e-9 from x0 = 0 with x* uniform in [-1,1] (splitmix64, --seed)
+        G.make_rhs(1, args.seed)
         xs = G.vec["xexact"].host()
         sol = {}
         for name, mode in (("f64", hip.CG_F64), ("mixed", hip.CG_MIXED)):
@@ -218,7 +272,11 @@ def main():
         sol["speedup_time_to_solution"] = round(sol["f64"]["seconds"] / sol["mixed"]["seconds"], 3)
         sol["iteration_delta"] = sol["mixed"]["iterations"] - sol["f64"]["iterations"]
         out["solve_1e-9"] = sol
+        sol["seed"] = args.seed
     G.close()
+
+    if not args.no_batch:
+        out["batched"] = batched_leg(hip, args.batch_count, log)
 
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

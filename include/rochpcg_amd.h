@@ -141,6 +141,26 @@ int hpcgx_set_residual_replacement(hpcgx_problem* p, int every);
 /* HPCG flop count of one CG iteration (ReportResults.cpp:73-89) */
 double hpcgx_flops_per_iteration(const hpcgx_problem* p);
 
+/* ---- batched solves (no reference counterpart) --------------------------- */
+
+/* Many independent problems of uneven sizes, each with its own hierarchy
+ * (dims: nprob triples nx, ny, nz).  Every operator of the V-cycle and of the
+ * CG runs as one launch per (operator, level, colour) across all problems,
+ * with per-problem reductions, scalars and convergence; one batched CG
+ * iteration is one hipGraph. */
+typedef struct hpcgx_batch hpcgx_batch;
+int hpcgx_batch_create(int nprob, const int* dims, int nlevels, int stores, void* stream, hpcgx_batch** out);
+int hpcgx_batch_destroy(hpcgx_batch* b);
+/* a non-owning handle on problem k (its vectors, rhs, per-problem operators);
+ * release it with hpcgx_problem_destroy, which then frees only the handle */
+int hpcgx_batch_problem(hpcgx_batch* b, int k, hpcgx_problem** out);
+/* threads per block of every batched kernel on `level` (64, 128 or 256) */
+int hpcgx_batch_set_block(hpcgx_batch* b, int level, int block);
+/* CG on every problem from its own x (b, x of each problem); niters, normr,
+ * normr0: arrays of nprob entries (may be NULL) */
+int hpcgx_batch_cg(hpcgx_batch* b, int mode, int max_iter, double tolerance, int use_graph, int* niters,
+                   double* normr, double* normr0);
+
 /* ---- timing and memory helpers ----------------------------------------- */
 
 /* average device time (ms) of `reps` back-to-back runs of one operator on

@@ -76,6 +76,11 @@ def _load():
         "hpcgx_set_shape": ([vp, i32, i32, i32, i32, i32], i32),
         "hpcgx_get_shape": ([vp, i32, i32, i32, P(i32), P(i32)], i32),
         "hpcgx_synchronize": ([vp], i32),
+        "hpcgx_batch_create": ([i32, vp, i32, i32, vp, P(vp)], i32),
+        "hpcgx_batch_destroy": ([vp], i32),
+        "hpcgx_batch_problem": ([vp, i32, P(vp)], i32),
+        "hpcgx_batch_set_block": ([vp, i32, i32], i32),
+        "hpcgx_batch_cg": ([vp, i32, i32, f64, i32, vp, vp, vp], i32),
         "hpcgx_malloc": ([P(vp), C.c_size_t], i32),
         "hpcgx_free": ([vp], i32),
         "hpcgx_memcpy_h2d": ([vp, vp, C.c_size_t], i32),
@@ -172,12 +177,14 @@ class Problem:
     reference, done on the device.  b = A*ones, x = 0, xexact = 1 initially.
     """
 
-    def __init__(self, nx, ny=None, nz=None, nlevels=4, stores=STORE_F64):
-        ny = nx if ny is None else ny
-        nz = nx if nz is None else nz
-        h = C.c_void_p()
-        _check(lib.hpcgx_problem_create(nx, ny, nz, nlevels, stores, None, C.byref(h)))
-        self.h = h.value
+    def __init__(self, nx, ny=None, nz=None, nlevels=4, stores=STORE_F64, handle=None):
+        if handle is None:
+            ny = nx if ny is None else ny
+            nz = nx if nz is None else nz
+            h = C.c_void_p()
+            _check(lib.hpcgx_problem_create(nx, ny, nz, nlevels, stores, None, C.byref(h)))
+            handle = h.value
+        self.h = handle
         self.nlevels = nlevels
         self.levels = []
         for lv in range(nlevels):
@@ -351,3 +358,44 @@ class Problem:
 
 def version() -> str:
     return lib.hpcgx_version().decode()
+
+
+class Batch:
+    """Independent problems of uneven sizes solved together (one launch per
+    operator, level and colour across all problems)."""
+
+    def __init__(self, dims, nlevels=4, stores=STORE_F64):
+        dims = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1, 3))
+        h = C.c_void_p()
+        _check(lib.hpcgx_batch_create(len(dims), dims.ctypes.data, nlevels, stores, None, C.byref(h)))
+        self.h = h.value
+        self.dims = dims
+        self.problems = []
+        for k in range(len(dims)):
+            ph = C.c_void_p()
+            _check(lib.hpcgx_batch_problem(self.h, k, C.byref(ph)))
+            self.problems.append(Problem(0, nlevels=nlevels, handle=ph.value))
+
+    def close(self):
+        if getattr(self, "h", None):
+            for p in self.problems:
+                p.close()   # frees the views only
+            lib.hpcgx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_block(self, level, block):
+        _check(lib.hpcgx_batch_set_block(self.h, level, block))
+
+    def cg(self, max_iter=50, tol=0.0, mode=CG_F64, use_graph=True):
+        n = len(self.problems)
+        ni = np.zeros(n, np.int32)
+        nr, nr0 = np.zeros(n), np.zeros(n)
+        _check(lib.hpcgx_batch_cg(self.h, mode, max_iter, tol, int(use_graph), ni.ctypes.data,
+                                  nr.ctypes.data, nr0.ctypes.data))
+        return dict(niters=ni, normr=nr, normr0=nr0)

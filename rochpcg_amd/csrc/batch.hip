@@ -1,0 +1,737 @@
+// batch.hip -- batched solves of many independent, unevenly sized HPCG
+// problems.  Each problem keeps its own hierarchy, vectors, reduction scratch
+// and CG scalars (a Problem); the V-cycle and the CG iteration are issued as
+// one launch per (operator, level, colour) covering every problem, so a whole
+// batched iteration is ~115 launches whatever the number of problems, and is
+// captured once as a hipGraph.  Blocks map to (problem, row block) through a
+// segment table in device memory; a problem's segments return at once after
+// its own convergence flag is set.
+//
+// Per-row arithmetic is shared with the single-problem kernels
+// (kernels.hpp), so each problem computes exactly what it would alone, up to
+// the grouping of the dot-product partials.
+#include "batch.hpp"
+#include "kernels.hpp"
+#include "setup.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+
+namespace hpcgx
+{
+
+__device__ __forceinline__ int find_seg(const BSeg* __restrict__ segs, int nseg, int64_t b)
+{
+    int lo = 0, hi = nseg - 1;
+    while(lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if(segs[mid].blk0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// block partials of one segment; the segment's last block folds them
+template <int BS>
+__device__ __forceinline__ void reduce_finish_seg(double v, const RedArgs& ra, int64_t lb, int64_t nblk)
+{
+    __shared__ double lds[BS / 64];
+    __shared__ int is_last;
+    const double bsum = block_sum<BS>(v, lds);
+    if(threadIdx.x == 0)
+    {
+        ra.partial[lb] = bsum;
+        __threadfence();
+        const unsigned t = atomicAdd(ra.ticket, 1u);
+        is_last = (t == (unsigned)(nblk - 1)) ? 1 : 0;
+    }
+    __syncthreads();
+    if(!is_last) return;
+    __threadfence();
+    double s = 0.0;
+    for(int64_t i = threadIdx.x; i < nblk; i += BS) s += __builtin_nontemporal_load(ra.partial + i);
+    __syncthreads();
+    s = block_sum<BS>(s, lds);
+    if(threadIdx.x == 0)
+    {
+        apply_op(ra.op, s, ra.sc, ra.hist, ra.hist_cap, ra.out);
+        atomicExch(ra.ticket, 0u);
+    }
+}
+
+#define BSEG()                                                \
+    const int64_t b_ = xcd_block(blockIdx.x, gridDim.x);      \
+    const BSeg& S = segs[find_seg(segs, nseg, b_)];           \
+    if(gated(S.gate)) return;                                 \
+    const int64_t lb = b_ - S.blk0;
+
+template <typename VT, typename XT, typename YT, bool DOT, int BS>
+__global__ __launch_bounds__(BS) void kb_spmv(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    const int64_t i = lb * BS + threadIdx.x;
+    double acc = 0.0;
+    if(i < S.size)
+    {
+        const int64_t row = S.off + i;
+        XT s[1];
+        row_spmv<VT, XT, 1>(S.m, row, S.col, (const VT*)S.val, (const XT*)S.r, s);
+        ((YT*)S.x)[row] = (YT)s[0];
+        if(DOT) acc = fmaT((double)((const XT*)S.r)[row], (double)s[0], acc);
+    }
+    if(DOT) reduce_finish_seg<BS>(acc, S.ra, lb, S.nblk);
+}
+
+template <typename VT, typename XT, typename RT, int BS>
+__global__ __launch_bounds__(BS) void kb_fwd_zero(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    const int64_t i = lb * BS + threadIdx.x;
+    if(i >= S.size) return;
+    row_fwd_zero<VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.dg, (const RT*)S.r, (XT*)S.x);
+}
+
+template <typename VT, typename XT, int BS>
+__global__ __launch_bounds__(BS) void kb_bwd_zero(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    const int64_t i = lb * BS + threadIdx.x;
+    if(i >= S.size) return;
+    row_bwd_zero<VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.dg, (XT*)S.x);
+}
+
+template <typename VT, typename XT, typename RT, int BS>
+__global__ __launch_bounds__(BS) void kb_fwd_full(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    const int64_t i = lb * BS + threadIdx.x;
+    if(i >= S.size) return;
+    if(S.t)
+        row_fwd_full<VT, XT, RT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
+                                          (XT*)S.t);
+    else
+        row_fwd_full<VT, XT, RT, false, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
+                                           nullptr);
+}
+
+template <typename VT, typename XT, int BS>
+__global__ __launch_bounds__(BS) void kb_bwd_full(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    const int64_t i = lb * BS + threadIdx.x;
+    if(i >= S.size) return;
+    row_bwd_full<VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.dg, (const XT*)S.t, (XT*)S.x);
+}
+
+template <typename VT, typename XT, typename RT, int BS>
+__global__ __launch_bounds__(BS) void kb_restrict(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    const int64_t k = lb * BS + threadIdx.x;
+    if(k >= S.size) return;
+    row_restrict<VT, XT, RT, XT>(S.m, k, S.col, (const VT*)S.val, S.f2c_fine, S.f2c_coarse, (const RT*)S.r,
+                                 (const XT*)S.x, (XT*)S.rc);
+}
+
+template <typename XT, int BS>
+__global__ __launch_bounds__(BS) void kb_prolong(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    const int64_t k = lb * BS + threadIdx.x;
+    if(k >= S.size) return;
+    const int64_t f = S.f2c_fine[k];
+    XT* x = (XT*)S.x;
+    x[f] = x[f] + ((const XT*)S.rc)[S.f2c_coarse[k]];
+}
+
+// vector segments: block lb of the segment handles elements [lb*chunk, ...)
+#define BCHUNK()                                                   \
+    const int64_t chunk_ = (S.size + S.nblk - 1) / S.nblk;         \
+    const int64_t e0 = lb * chunk_;                                \
+    const int64_t e1 = (e0 + chunk_ < S.size) ? e0 + chunk_ : S.size;
+
+template <typename XT, typename YT, int BS>
+__global__ __launch_bounds__(BS) void kb_dot(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    BCHUNK();
+    const XT* a = (const XT*)S.a;
+    const YT* bb = (const YT*)S.b;
+    double acc = 0.0;
+    for(int64_t i = e0 + threadIdx.x; i < e1; i += BS) acc = fmaT((double)bb[i], (double)a[i], acc);
+    reduce_finish_seg<BS>(acc, S.ra, lb, S.nblk);
+}
+
+template <typename ZT, int BS>
+__global__ __launch_bounds__(BS) void kb_cg_p(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    BCHUNK();
+    const ZT* z = (const ZT*)S.a;
+    double* p = (double*)S.w;
+    const bool first = S.ra.sc->iter == 0;
+    const double beta = S.ra.sc->beta;
+    for(int64_t i = e0 + threadIdx.x; i < e1; i += BS) p[i] = first ? (double)z[i] : fmaT(beta, p[i], (double)z[i]);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void kb_cg_xr(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    BCHUNK();
+    const double* p = (const double*)S.a;
+    const double* Ap = (const double*)S.b;
+    double* x = (double*)S.x;
+    double* r = (double*)S.w;
+    const double alpha = S.ra.sc->alpha;
+    double acc = 0.0;
+    for(int64_t i = e0 + threadIdx.x; i < e1; i += BS)
+    {
+        const double rv = fmaT(-alpha, Ap[i], r[i]);
+        r[i] = rv;
+        x[i] = fmaT(alpha, p[i], x[i]);
+        acc = fmaT(rv, rv, acc);
+    }
+    reduce_finish_seg<BS>(acc, S.ra, lb, S.nblk);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void kb_copy(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    BCHUNK();
+    const double* a = (const double*)S.a;
+    double* w = (double*)S.w;
+    for(int64_t i = e0 + threadIdx.x; i < e1; i += BS) w[i] = a[i];
+}
+
+// w = a + beta*b (ComputeWAXPBY's alpha == 1 form)
+template <int BS>
+__global__ __launch_bounds__(BS) void kb_waxpby(const BSeg* __restrict__ segs, int nseg, double beta)
+{
+    BSEG();
+    BCHUNK();
+    const double* a = (const double*)S.a;
+    const double* bb = (const double*)S.b;
+    double* w = (double*)S.w;
+    for(int64_t i = e0 + threadIdx.x; i < e1; i += BS) w[i] = fmaT(beta, bb[i], a[i]);
+}
+#undef BSEG
+#undef BCHUNK
+
+// ---------------------------------------------------------------------------
+// launching a planned record
+
+template <int BS>
+static void launch_bs(const BLaunch& L, hipStream_t s)
+{
+    const dim3 g((unsigned)L.nblocks), b(BS);
+    const BSeg* T = L.segs;
+    const int n = L.nseg;
+    const bool f64 = L.prec == PREC_F64;
+    switch(L.kind)
+    {
+    case BK_SPMV:
+        // outer SpMV: f64 vectors over either store; MG-internal SpMV is not batched
+        if(f64)
+        {
+            if(L.dot)
+                hipLaunchKernelGGL((kb_spmv<double, double, double, true, BS>), g, b, 0, s, T, n);
+            else
+                hipLaunchKernelGGL((kb_spmv<double, double, double, false, BS>), g, b, 0, s, T, n);
+        }
+        else
+        {
+            if(L.dot)
+                hipLaunchKernelGGL((kb_spmv<__half, double, double, true, BS>), g, b, 0, s, T, n);
+            else
+                hipLaunchKernelGGL((kb_spmv<__half, double, double, false, BS>), g, b, 0, s, T, n);
+        }
+        break;
+    case BK_FWD_ZERO:
+        if(f64)
+            hipLaunchKernelGGL((kb_fwd_zero<double, double, double, BS>), g, b, 0, s, T, n);
+        else if(L.r_f64)
+            hipLaunchKernelGGL((kb_fwd_zero<__half, float, double, BS>), g, b, 0, s, T, n);
+        else
+            hipLaunchKernelGGL((kb_fwd_zero<__half, float, float, BS>), g, b, 0, s, T, n);
+        break;
+    case BK_BWD_ZERO:
+        if(f64)
+            hipLaunchKernelGGL((kb_bwd_zero<double, double, BS>), g, b, 0, s, T, n);
+        else
+            hipLaunchKernelGGL((kb_bwd_zero<__half, float, BS>), g, b, 0, s, T, n);
+        break;
+    case BK_FWD_FULL:
+        if(f64)
+            hipLaunchKernelGGL((kb_fwd_full<double, double, double, BS>), g, b, 0, s, T, n);
+        else if(L.r_f64)
+            hipLaunchKernelGGL((kb_fwd_full<__half, float, double, BS>), g, b, 0, s, T, n);
+        else
+            hipLaunchKernelGGL((kb_fwd_full<__half, float, float, BS>), g, b, 0, s, T, n);
+        break;
+    case BK_BWD_FULL:
+        if(f64)
+            hipLaunchKernelGGL((kb_bwd_full<double, double, BS>), g, b, 0, s, T, n);
+        else
+            hipLaunchKernelGGL((kb_bwd_full<__half, float, BS>), g, b, 0, s, T, n);
+        break;
+    case BK_RESTRICT:
+        if(f64)
+            hipLaunchKernelGGL((kb_restrict<double, double, double, BS>), g, b, 0, s, T, n);
+        else if(L.r_f64)
+            hipLaunchKernelGGL((kb_restrict<__half, float, double, BS>), g, b, 0, s, T, n);
+        else
+            hipLaunchKernelGGL((kb_restrict<__half, float, float, BS>), g, b, 0, s, T, n);
+        break;
+    case BK_PROLONG:
+        if(f64)
+            hipLaunchKernelGGL((kb_prolong<double, BS>), g, b, 0, s, T, n);
+        else
+            hipLaunchKernelGGL((kb_prolong<float, BS>), g, b, 0, s, T, n);
+        break;
+    case BK_DOT:
+        if(!L.bf32)
+            hipLaunchKernelGGL((kb_dot<double, double, BS>), g, b, 0, s, T, n);
+        else
+            hipLaunchKernelGGL((kb_dot<double, float, BS>), g, b, 0, s, T, n);
+        break;
+    case BK_CG_P:
+        if(!L.bf32)
+            hipLaunchKernelGGL((kb_cg_p<double, BS>), g, b, 0, s, T, n);
+        else
+            hipLaunchKernelGGL((kb_cg_p<float, BS>), g, b, 0, s, T, n);
+        break;
+    case BK_CG_XR: hipLaunchKernelGGL((kb_cg_xr<BS>), g, b, 0, s, T, n); break;
+    case BK_COPY: hipLaunchKernelGGL((kb_copy<BS>), g, b, 0, s, T, n); break;
+    case BK_WAXPBY: hipLaunchKernelGGL((kb_waxpby<BS>), g, b, 0, s, T, n, L.beta); break;
+    }
+}
+
+static int launch(const BLaunch& L, hipStream_t s)
+{
+    if(L.nblocks == 0) return 0;
+    switch(L.block)
+    {
+    case 64: launch_bs<64>(L, s); break;
+    case 128: launch_bs<128>(L, s); break;
+    default: launch_bs<256>(L, s); break;
+    }
+    HPCGX_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// planning
+
+struct Planner
+{
+    Batch& B;
+    std::vector<BLaunch>* out;
+    int prec;
+    int next_bf32 = 0;   // the next vector launch reads an f32 z
+
+    // append a launch whose segments are built by `fill(k, seg) -> bool`
+    template <typename Fill>
+    int add(int kind, int block, bool vector_kind, int r_f64, int dot, double beta, Fill&& fill)
+    {
+        std::vector<BSeg> segs;
+        int64_t nb = 0;
+        for(size_t k = 0; k < B.probs.size(); ++k)
+        {
+            BSeg sg;
+            std::memset(&sg, 0, sizeof(sg));
+            sg.gate = &B.probs[k]->sc->done;
+            if(!fill((int)k, sg) || sg.size <= 0) continue;
+            if(vector_kind)
+            {
+                int64_t g = ceil_div(sg.size, (int64_t)block * 4);
+                sg.nblk = std::max<int64_t>(1, std::min<int64_t>(g, RED_BLOCKS));
+            }
+            else
+            {
+                sg.nblk = ceil_div(sg.size, block);
+            }
+            if(sg.ra.partial && sg.nblk > B.probs[k]->partial_cap)
+            {
+                set_error("batched reduction exceeds the partial capacity");
+                return -2;
+            }
+            sg.blk0 = nb;
+            nb += sg.nblk;
+            segs.push_back(sg);
+        }
+        BLaunch L;
+        std::memset(&L, 0, sizeof(L));
+        L.kind = kind;
+        L.prec = prec;
+        L.r_f64 = r_f64;
+        L.dot = dot;
+        L.block = block;
+        L.nseg = (int)segs.size();
+        L.nblocks = nb;
+        L.beta = beta;
+        L.bf32 = next_bf32;
+        next_bf32 = 0;
+        if(!segs.empty())
+        {
+            BSeg* d = nullptr;
+            HPCGX_HIP(hipMalloc(&d, sizeof(BSeg) * segs.size()));
+            HPCGX_HIP(hipMemcpy(d, segs.data(), sizeof(BSeg) * segs.size(), hipMemcpyHostToDevice));
+            B.tables.push_back(d);
+            L.segs = d;
+            out->push_back(L);
+        }
+        return 0;
+    }
+
+    RedArgs red(Problem* P, int op)
+    {
+        RedArgs ra;
+        ra.partial = P->partial;
+        ra.ticket = P->ticket;
+        ra.op = op;
+        ra.sc = P->sc;
+        ra.hist = P->hist;
+        ra.hist_cap = P->hist_cap;
+        ra.out = P->dout;
+        ra.cap = P->partial_cap;
+        return ra;
+    }
+
+    const void* vals(const Level& L) { return prec == PREC_F64 ? (const void*)L.ell_val : (const void*)L.ell_h; }
+    LevelVecs* vecs(Problem* P) { return prec == PREC_F64 ? P->v64 : P->v32; }
+    void* z(Problem* P) { return prec == PREC_F64 ? (void*)P->z64 : (void*)P->z32; }
+
+    int max_colors(int l)
+    {
+        int n = 0;
+        for(Problem* P : B.probs) n = std::max(n, P->L[l].ncolors);
+        return n;
+    }
+
+    // the V-cycle from level l (ComputeMG.cpp:64-102), all problems in lockstep
+    int mg(int l)
+    {
+        const int bs = B.block[l];
+        const bool last = (l + 1 == B.nlevels);
+        const int r64 = (l == 0) ? 1 : (prec == PREC_F64);
+        auto level_seg = [&](int k, BSeg& s, int c) {
+            Problem* P = B.probs[k];
+            const Level& L = P->L[l];
+            if(c >= L.ncolors) return false;
+            s.m = L.m;
+            s.off = L.offsets[c];
+            s.size = L.sizes[c];
+            s.col = L.ell_col;
+            s.val = vals(L);
+            s.dg = L.diag_idx;
+            s.r = (l == 0) ? (const void*)P->r : vecs(P)[l].r;
+            s.x = (l == 0) ? z(P) : vecs(P)[l].x;
+            return true;
+        };
+        const int C = max_colors(l);
+        for(int c = 0; c < C; ++c)
+            HPCGX_TRY(add(BK_FWD_ZERO, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) { return level_seg(k, s, c); }));
+        for(int c = C - 2; c >= 0; --c)
+            HPCGX_TRY(add(BK_BWD_ZERO, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
+                return c <= B.probs[k]->L[l].ncolors - 2 && level_seg(k, s, c);
+            }));
+        if(last) return 0;
+        HPCGX_TRY(add(BK_RESTRICT, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
+            Problem* P = B.probs[k];
+            const Level& L = P->L[l];
+            s.m = L.m;
+            s.size = L.mc;
+            s.col = L.ell_col;
+            s.val = vals(L);
+            s.f2c_fine = L.f2c_fine;
+            s.f2c_coarse = L.f2c_coarse;
+            s.r = (l == 0) ? (const void*)P->r : vecs(P)[l].r;
+            s.x = (l == 0) ? z(P) : vecs(P)[l].x;
+            s.rc = vecs(P)[l + 1].r;
+            return true;
+        }));
+        HPCGX_TRY(mg(l + 1));
+        HPCGX_TRY(add(BK_PROLONG, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
+            Problem* P = B.probs[k];
+            const Level& L = P->L[l];
+            s.size = L.mc;
+            s.f2c_fine = L.f2c_fine;
+            s.f2c_coarse = L.f2c_coarse;
+            s.rc = vecs(P)[l + 1].x;
+            s.x = (l == 0) ? z(P) : vecs(P)[l].x;
+            return true;
+        }));
+        for(int c = 0; c < C; ++c)
+            HPCGX_TRY(add(BK_FWD_FULL, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
+                if(!level_seg(k, s, c)) return false;
+                s.t = (c < B.probs[k]->L[l].ncolors - 1) ? vecs(B.probs[k])[l].t : nullptr;
+                return true;
+            }));
+        for(int c = C - 2; c >= 0; --c)
+            HPCGX_TRY(add(BK_BWD_FULL, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
+                if(!(c <= B.probs[k]->L[l].ncolors - 2 && level_seg(k, s, c))) return false;
+                s.t = vecs(B.probs[k])[l].t;
+                return true;
+            }));
+        return 0;
+    }
+
+    int outer_spmv(bool dot, bool from_x, bool into_tmp)
+    {
+        return add(BK_SPMV, B.block[0], false, 1, dot ? 1 : 0, 0.0, [&](int k, BSeg& s) {
+            Problem* P = B.probs[k];
+            const Level& L = P->L[0];
+            s.m = L.m;
+            s.off = 0;
+            s.size = L.m;
+            s.col = L.ell_col;
+            s.val = vals(L);
+            s.r = from_x ? (const void*)P->x : (const void*)P->p;
+            s.x = into_tmp ? (void*)P->tmp : (void*)P->Ap;
+            if(dot) s.ra = red(P, OP_PAP);
+            return true;
+        });
+    }
+
+    int vec(int kind, int op, std::function<void(Problem*, BSeg&)> set, double beta = 0.0)
+    {
+        return add(kind, RED_THREADS, true, 1, 0, beta, [&](int k, BSeg& s) {
+            Problem* P = B.probs[k];
+            s.size = P->L[0].m;
+            set(P, s);
+            if(op != OP_NONE) s.ra = red(P, op);
+            return true;
+        });
+    }
+
+    int true_residual(int op)
+    {
+        HPCGX_TRY(outer_spmv(false, true, true));
+        HPCGX_TRY(vec(BK_WAXPBY, OP_NONE, [](Problem* P, BSeg& s) {
+            s.a = P->b;
+            s.b = P->tmp;
+            s.w = P->r;
+        }, -1.0));
+        return vec(BK_DOT, op, [](Problem* P, BSeg& s) {
+            s.a = P->r;
+            s.b = P->r;
+        });
+    }
+};
+
+static int plan(Batch& B)
+{
+    for(int mode = 0; mode < 2; ++mode)
+    {
+        const bool mixed = mode == CG_MIXED;
+        if(mixed ? !B.probs[0]->has_mixed : !B.probs[0]->has_f64) continue;
+        const int prec = mixed ? PREC_MIXED : PREC_F64;
+        // preamble (CG.cpp:133-137): p = x; Ap = A p; r = b - Ap; ||r||
+        Planner pre{B, &B.pre[mode], prec};
+        HPCGX_TRY(pre.vec(BK_COPY, OP_NONE, [](Problem* P, BSeg& s) {
+            s.a = P->x;
+            s.w = P->p;
+        }));
+        HPCGX_TRY(pre.outer_spmv(false, false, false));
+        HPCGX_TRY(pre.vec(BK_WAXPBY, OP_NONE, [](Problem* P, BSeg& s) {
+            s.a = P->b;
+            s.b = P->Ap;
+            s.w = P->r;
+        }, -1.0));
+        HPCGX_TRY(pre.vec(BK_DOT, OP_NORMR0, [](Problem* P, BSeg& s) {
+            s.a = P->r;
+            s.b = P->r;
+        }));
+        // one iteration (CG.cpp:150-171)
+        Planner it{B, &B.iter[mode], prec};
+        HPCGX_TRY(it.mg(0));
+        it.next_bf32 = mixed;   // r (f64) . z (f32 in the mixed mode)
+        HPCGX_TRY(it.vec(BK_DOT, OP_RTZ, [mixed](Problem* P, BSeg& s) {
+            s.a = P->r;
+            s.b = mixed ? (const void*)P->z32 : (const void*)P->z64;
+        }));
+        it.next_bf32 = mixed;
+        HPCGX_TRY(it.vec(BK_CG_P, OP_NONE, [mixed](Problem* P, BSeg& s) {
+            s.a = mixed ? (const void*)P->z32 : (const void*)P->z64;
+            s.w = P->p;
+            s.ra.sc = P->sc;
+        }));
+        HPCGX_TRY(it.outer_spmv(true, false, false));
+        HPCGX_TRY(it.vec(BK_CG_XR, OP_NORMR, [](Problem* P, BSeg& s) {
+            s.a = P->p;
+            s.b = P->Ap;
+            s.x = P->x;
+            s.w = P->r;
+        }));
+    }
+    B.planned = true;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+
+static void free_plans(Batch& B)
+{
+    for(int g = 0; g < 2; ++g)
+    {
+        if(B.gexec[g]) (void)hipGraphExecDestroy(B.gexec[g]);
+        if(B.graph[g]) (void)hipGraphDestroy(B.graph[g]);
+        B.gexec[g] = nullptr;
+        B.graph[g] = nullptr;
+        B.pre[g].clear();
+        B.iter[g].clear();
+    }
+    for(BSeg* t : B.tables) (void)hipFree(t);
+    B.tables.clear();
+    B.planned = false;
+}
+
+int batch_create(Batch*& out, int nprob, const int* dims, int nlevels, bool with_f64, bool with_mixed, hipStream_t s)
+{
+    out = nullptr;
+    if(nprob < 1)
+    {
+        set_error("a batch needs at least one problem");
+        return -2;
+    }
+    Batch* B = new Batch;
+    B->nlevels = nlevels;
+    if(s)
+        B->s = s;
+    else
+    {
+        if(hipStreamCreateWithFlags(&B->s, hipStreamNonBlocking) != hipSuccess)
+        {
+            set_error("cannot create the batch stream");
+            delete B;
+            return -1;
+        }
+        B->own_stream = true;
+    }
+    for(int k = 0; k < nprob; ++k)
+    {
+        Problem* P = nullptr;
+        int rc = problem_create(P, dims[3 * k], dims[3 * k + 1], dims[3 * k + 2], nlevels, with_f64, with_mixed, B->s);
+        if(rc)
+        {
+            batch_destroy(B);
+            return rc;
+        }
+        B->probs.push_back(P);
+    }
+    out = B;
+    return 0;
+}
+
+void batch_destroy(Batch* B)
+{
+    if(!B) return;
+    if(B->s) (void)hipStreamSynchronize(B->s);
+    free_plans(*B);
+    for(Problem* P : B->probs) problem_destroy(P);
+    if(B->own_stream && B->s) (void)hipStreamDestroy(B->s);
+    delete B;
+}
+
+int batch_set_block(Batch& B, int level, int block)
+{
+    if(level < 0 || level >= B.nlevels || (block != 64 && block != 128 && block != 256))
+    {
+        set_error("invalid level or block size");
+        return -2;
+    }
+    B.block[level] = block;
+    HPCGX_HIP(hipStreamSynchronize(B.s));
+    free_plans(B);
+    return 0;
+}
+
+static int run_list(const std::vector<BLaunch>& v, hipStream_t s)
+{
+    for(const BLaunch& L : v) HPCGX_TRY(launch(L, s));
+    return 0;
+}
+
+int batch_cg(Batch& B, int mode, int max_iter, double tol, bool use_graph, int* niters, double* normr, double* normr0)
+{
+    if(mode != CG_F64 && mode != CG_MIXED)
+    {
+        set_error("invalid CG mode");
+        return -2;
+    }
+    if(!B.planned) HPCGX_TRY(plan(B));
+    if(B.iter[mode].empty())
+    {
+        set_error("the batch was built without the store this mode needs");
+        return -2;
+    }
+    if(max_iter < 0 || max_iter > B.probs[0]->hist_cap)
+    {
+        set_error("max_iter out of range");
+        return -2;
+    }
+    // per-problem scalars (cg_begin's reset)
+    for(Problem* P : B.probs)
+    {
+        std::memset(&P->h_sc, 0, sizeof(P->h_sc));
+        P->h_sc.tol = tol;
+        P->h_sc.max_iter = max_iter;
+        HPCGX_HIP(hipMemcpyAsync(P->sc, &P->h_sc, sizeof(CGScalars), hipMemcpyHostToDevice, B.s));
+    }
+    HPCGX_HIP(hipStreamSynchronize(B.s));
+    HPCGX_TRY(run_list(B.pre[mode], B.s));
+    if(use_graph && !B.gexec[mode])
+    {
+        HPCGX_HIP(hipStreamBeginCapture(B.s, hipStreamCaptureModeThreadLocal));
+        const int rc = run_list(B.iter[mode], B.s);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(B.s, &g);
+        if(rc) return rc;
+        if(e != hipSuccess)
+        {
+            set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            return -1;
+        }
+        B.graph[mode] = g;
+        HPCGX_HIP(hipGraphInstantiate(&B.gexec[mode], g, nullptr, nullptr, 0));
+    }
+    const int check = 10;
+    std::vector<CGScalars> h(B.probs.size());
+    for(int it = 0; it < max_iter;)
+    {
+        const int n = std::min(check, max_iter - it);
+        for(int i = 0; i < n; ++i)
+        {
+            if(use_graph)
+                HPCGX_HIP(hipGraphLaunch(B.gexec[mode], B.s));
+            else
+                HPCGX_TRY(run_list(B.iter[mode], B.s));
+        }
+        it += n;
+        if(tol <= 0.0 || it >= max_iter) continue;
+        bool all = true;
+        for(size_t k = 0; k < B.probs.size(); ++k)
+            HPCGX_HIP(hipMemcpyAsync(&h[k], B.probs[k]->sc, sizeof(CGScalars), hipMemcpyDeviceToHost, B.s));
+        HPCGX_HIP(hipStreamSynchronize(B.s));
+        for(auto& v : h) all = all && v.done;
+        if(all) break;
+    }
+    for(size_t k = 0; k < B.probs.size(); ++k)
+        HPCGX_HIP(hipMemcpyAsync(&h[k], B.probs[k]->sc, sizeof(CGScalars), hipMemcpyDeviceToHost, B.s));
+    HPCGX_HIP(hipStreamSynchronize(B.s));
+    for(size_t k = 0; k < B.probs.size(); ++k)
+    {
+        if(niters) niters[k] = h[k].iter;
+        if(normr) normr[k] = h[k].normr;
+        if(normr0) normr0[k] = h[k].normr0;
+    }
+    return 0;
+}
+
+}  // namespace hpcgx

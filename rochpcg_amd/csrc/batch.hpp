@@ -1,0 +1,88 @@
+// batch.hpp -- many independent HPCG problems of uneven sizes solved together:
+// each problem keeps its own hierarchy (a Problem), and every operator of
+// the V-cycle and the CG runs as ONE launch per (operator, level, colour)
+// across all problems.  Blocks find their (problem, row block) segment in a
+// device table; reductions, CG scalars and convergence flags stay per problem.
+#pragma once
+#include "solver.hpp"
+
+namespace hpcgx
+{
+
+// One contiguous row range of one problem in one launch.
+struct BSeg
+{
+    int64_t blk0;      // first (logical) block of the segment in the launch
+    int64_t nblk;      // blocks of the segment
+    int64_t m;         // rows of the problem's level (ELL stride)
+    int64_t off;       // first row (or element) of the segment
+    int64_t size;      // rows (or elements) in the segment
+    const int32_t* col;
+    const void* val;
+    const int32_t* dg;
+    const void* r;     // rhs / first vector operand
+    void* x;           // solution / in-out operand
+    void* t;           // SymGS partial sums (null: do not keep them)
+    const int32_t* f2c_fine;
+    const int32_t* f2c_coarse;
+    void* rc;          // coarse vector (restriction output / prolongation input)
+    const void* a;     // vector kernels: operands
+    const void* b;
+    void* w;
+    const int* gate;   // the problem's convergence flag
+    RedArgs ra;        // the problem's reduction scratch and scalars
+};
+
+enum BKind : int
+{
+    BK_SPMV = 0,      // y = A x (r -> x), optional p'Ap
+    BK_FWD_ZERO,
+    BK_BWD_ZERO,
+    BK_FWD_FULL,
+    BK_BWD_FULL,
+    BK_RESTRICT,
+    BK_PROLONG,
+    BK_DOT,           // reduction of a . b
+    BK_CG_P,          // p = z (+ beta p)
+    BK_CG_XR,         // x += alpha p, r -= alpha Ap, ||r||
+    BK_COPY,          // w = a
+    BK_WAXPBY,        // w = alpha a + beta b (alpha == 1 form only: b - A x)
+};
+
+struct BLaunch
+{
+    int kind;
+    int prec;         // PREC_F64 / PREC_MIXED (value and vector types)
+    int r_f64;        // rhs is f64 (mixed level 0)
+    int dot;          // SpMV: fuse p'Ap
+    int bf32;         // vector kernels: the second operand (z) is f32
+    int block;        // threads per block
+    int nseg;
+    int64_t nblocks;
+    BSeg* segs;       // device table
+    double alpha, beta;
+};
+
+struct Batch
+{
+    std::vector<Problem*> probs;
+    int nlevels = 0;
+    hipStream_t s = nullptr;
+    bool own_stream = false;
+    int block[MAX_LEVELS] = {256, 256, 256, 256, 256, 256, 256, 256};
+    std::vector<BSeg*> tables;              // device allocations of the plans
+    std::vector<BLaunch> pre[2], iter[2];   // preamble and one iteration, per CG mode
+    bool planned = false;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int host_iter = 0;
+    int mode = 0;
+};
+
+int batch_create(Batch*& B, int nprob, const int* dims, int nlevels, bool with_f64, bool with_mixed, hipStream_t s);
+void batch_destroy(Batch* B);
+int batch_set_block(Batch& B, int level, int block);
+int batch_cg(Batch& B, int mode, int max_iter, double tol, bool use_graph, int* niters, double* normr,
+             double* normr0);
+
+}  // namespace hpcgx

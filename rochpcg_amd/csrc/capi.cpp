@@ -5,6 +5,7 @@
 #include <cstring>
 #include <string>
 
+#include "batch.hpp"
 #include "ops.hpp"
 #include "setup.hpp"
 #include "solver.hpp"
@@ -21,6 +22,12 @@ using namespace hpcgx;
 struct hpcgx_problem
 {
     Problem* P;
+    bool owned;   // false for a view of one problem of a batch
+};
+
+struct hpcgx_batch
+{
+    Batch* B;
 };
 
 #define CHECK_P(p)                                                                                                     \
@@ -67,14 +74,14 @@ int hpcgx_problem_create(int nx, int ny, int nz, int nlevels, int stores, void* 
     int rc = problem_create(P, nx, ny, nz, nlevels, (stores & HPCGX_STORE_F64) != 0, (stores & HPCGX_STORE_MIXED) != 0,
                             (hipStream_t)stream);
     if(rc) return rc;
-    *out = new hpcgx_problem{P};
+    *out = new hpcgx_problem{P, true};
     return 0;
 }
 
 int hpcgx_problem_destroy(hpcgx_problem* p)
 {
     if(!p) return 0;
-    problem_destroy(p->P);
+    if(p->owned) problem_destroy(p->P);
     delete p;
     return 0;
 }
@@ -476,6 +483,61 @@ int hpcgx_device_count(int* count)
     *count = 0;
     HPCGX_HIP(hipGetDeviceCount(count));
     return 0;
+}
+
+int hpcgx_batch_create(int nprob, const int* dims, int nlevels, int stores, void* stream, hpcgx_batch** out)
+{
+    if(!out || !dims)
+    {
+        set_error("null argument");
+        return -2;
+    }
+    *out = nullptr;
+    Batch* B = nullptr;
+    HPCGX_TRY(batch_create(B, nprob, dims, nlevels, (stores & HPCGX_STORE_F64) != 0,
+                           (stores & HPCGX_STORE_MIXED) != 0, (hipStream_t)stream));
+    *out = new hpcgx_batch{B};
+    return 0;
+}
+
+int hpcgx_batch_destroy(hpcgx_batch* b)
+{
+    if(!b) return 0;
+    batch_destroy(b->B);
+    delete b;
+    return 0;
+}
+
+int hpcgx_batch_problem(hpcgx_batch* b, int k, hpcgx_problem** out)
+{
+    if(!b || !b->B || !out || k < 0 || k >= (int)b->B->probs.size())
+    {
+        set_error("invalid batch or problem index");
+        return -2;
+    }
+    *out = new hpcgx_problem{b->B->probs[k], false};
+    return 0;
+}
+
+int hpcgx_batch_set_block(hpcgx_batch* b, int level, int block)
+{
+    if(!b || !b->B)
+    {
+        set_error("null batch");
+        return -2;
+    }
+    return batch_set_block(*b->B, level, block);
+}
+
+int hpcgx_batch_cg(hpcgx_batch* b, int mode, int max_iter, double tolerance, int use_graph, int* niters,
+                   double* normr, double* normr0)
+{
+    if(!b || !b->B)
+    {
+        set_error("null batch");
+        return -2;
+    }
+    return batch_cg(*b->B, mode, max_iter, tolerance, use_graph != 0, niters, normr, normr0);
 }
 
 const char* hpcgx_last_error(void) { return last_error(); }

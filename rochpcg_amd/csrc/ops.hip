@@ -17,216 +17,13 @@
 //   dot / waxpby / fused waxpby-dot               ComputeDotProduct.cpp:63-238,
 //                                                 ComputeWAXPBY.cpp:63-240
 #include "common.hpp"
+#include "kernels.hpp"
 #include "ops.hpp"
 
 #include <type_traits>
 
 namespace hpcgx
 {
-
-// ---------------------------------------------------------------------------
-// scalar helpers
-__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-
-template <typename T>
-__device__ __forceinline__ T cvt(double v) { return (T)v; }
-template <typename T>
-__device__ __forceinline__ T cvt(float v) { return (T)v; }
-template <typename T>
-__device__ __forceinline__ T cvt(__half v) { return (T)__half2float(v); }
-
-__device__ __forceinline__ bool gated(const int* gate) { return gate != nullptr && *gate != 0; }
-
-// ---------------------------------------------------------------------------
-// deterministic block-partials reduction; the last block folds the partials
-// in index order, applies `op` and re-arms the ticket.
-__device__ void apply_op(int op, double v, CGScalars* sc, double* hist, int hist_cap, double* out)
-{
-    switch(op)
-    {
-    case OP_STORE: *out = v; break;
-    case OP_NORMR0:
-    {
-        const double nr = sqrt(v);
-        sc->normr = nr;
-        sc->normr0 = nr;
-        sc->iter = 0;
-        if(hist && hist_cap >= 0) hist[0] = nr;
-        sc->done = (!(nr / nr > sc->tol) || sc->max_iter <= 0) ? 1 : 0;
-        break;
-    }
-    case OP_RTZ:
-        sc->oldrtz = sc->rtz;
-        sc->rtz = v;
-        sc->beta = (sc->iter > 0) ? v / sc->oldrtz : 0.0;
-        break;
-    case OP_PAP:
-        sc->pAp = v;
-        sc->alpha = sc->rtz / v;
-        break;
-    case OP_NORMR:
-    {
-        const double nr = sqrt(v);
-        sc->normr = nr;
-        const int it = sc->iter + 1;
-        sc->iter = it;
-        if(hist && it <= hist_cap) hist[it] = nr;
-        sc->done = (!(nr / sc->normr0 > sc->tol) || it >= sc->max_iter) ? 1 : 0;
-        break;
-    }
-    case OP_NORMR_SET:
-    {
-        const double nr = sqrt(v);
-        sc->normr = nr;
-        const int it = sc->iter;
-        if(hist && it <= hist_cap) hist[it] = nr;
-        sc->done = (!(nr / sc->normr0 > sc->tol) || it >= sc->max_iter) ? 1 : 0;
-        break;
-    }
-    default: break;
-    }
-}
-
-template <int BS>
-__device__ __forceinline__ void reduce_finish(double v, const RedArgs& ra)
-{
-    __shared__ double lds[BS / 64];
-    __shared__ int is_last;
-    const double bsum = block_sum<BS>(v, lds);
-    if(threadIdx.x == 0)
-    {
-        ra.partial[blockIdx.x] = bsum;
-        __threadfence();
-        const unsigned t = atomicAdd(ra.ticket, 1u);
-        is_last = (t == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if(!is_last) return;
-    __threadfence();
-    double s = 0.0;
-    for(unsigned i = threadIdx.x; i < gridDim.x; i += BS) s += __builtin_nontemporal_load(ra.partial + i);
-    __syncthreads();
-    s = block_sum<BS>(s, lds);
-    if(threadIdx.x == 0)
-    {
-        apply_op(ra.op, s, ra.sc, ra.hist, ra.hist_cap, ra.out);
-        atomicExch(ra.ticket, 0u);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Vector loads of R consecutive rows of one ELL slot.  A thread owns rows
-// [row, row+R) (row % R == 0, m % R == 0), so a wave issues one contiguous
-// 64*R-element segment per slot: dwordx2/x4 loads instead of R dword loads.
-template <typename T, int R>
-struct Ext;
-#define HPCGX_EXT(T, R) \
-    template <>         \
-    struct Ext<T, R>    \
-    {                   \
-        typedef T type __attribute__((ext_vector_type(R))); \
-    };
-HPCGX_EXT(int32_t, 2)
-HPCGX_EXT(int32_t, 4)
-HPCGX_EXT(double, 2)
-HPCGX_EXT(double, 4)
-HPCGX_EXT(float, 2)
-HPCGX_EXT(float, 4)
-HPCGX_EXT(uint16_t, 2)
-HPCGX_EXT(uint16_t, 4)
-#undef HPCGX_EXT
-
-template <int R, typename T>
-__device__ __forceinline__ void ld_nt(const T* p, T (&o)[R])
-{
-    if constexpr(R == 1)
-        o[0] = ntload(p);
-    else
-    {
-        typedef typename Ext<T, R>::type V;
-        const V w = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
-#pragma unroll
-        for(int k = 0; k < R; ++k) o[k] = w[k];
-    }
-}
-
-template <int R>
-__device__ __forceinline__ void ld_nt(const __half* p, __half (&o)[R])
-{
-    if constexpr(R == 1)
-        o[0] = ntload(p);
-    else
-    {
-        typedef typename Ext<uint16_t, R>::type V;
-        const V w = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
-#pragma unroll
-        for(int k = 0; k < R; ++k) o[k] = __ushort_as_half(w[k]);
-    }
-}
-
-template <int R, typename T>
-__device__ __forceinline__ void ld(const T* p, T (&o)[R])
-{
-    if constexpr(R == 1)
-        o[0] = *p;
-    else
-    {
-        typedef typename Ext<T, R>::type V;
-        const V w = *reinterpret_cast<const V*>(p);
-#pragma unroll
-        for(int k = 0; k < R; ++k) o[k] = w[k];
-    }
-}
-
-template <int R, typename T>
-__device__ __forceinline__ void st(T* p, const T (&v)[R])
-{
-    if constexpr(R == 1)
-        *p = v[0];
-    else
-    {
-        typedef typename Ext<T, R>::type V;
-        V w;
-#pragma unroll
-        for(int k = 0; k < R; ++k) w[k] = v[k];
-        *reinterpret_cast<V*>(p) = w;
-    }
-}
-
-// r may be f64 while x is f32 (mixed level 0): load in RT, convert.
-template <int R, typename XT, typename RT>
-__device__ __forceinline__ void ld_cvt(const RT* p, XT (&o)[R])
-{
-    RT t[R];
-    ld<R>(p, t);
-#pragma unroll
-    for(int k = 0; k < R; ++k) o[k] = (XT)t[k];
-}
-
-template <int R, typename XT, typename YT>
-__device__ __forceinline__ void st_cvt(YT* p, const XT (&v)[R])
-{
-    YT t[R];
-#pragma unroll
-    for(int k = 0; k < R; ++k) t[k] = (YT)v[k];
-    st<R>(p, t);
-}
-
-// ---------------------------------------------------------------------------
-// Row tiles of a whole-level operator: tile t -> (slab j = t / ncolors,
-// colour c = t % ncolors), rows offsets[c] + j*BS*R + [0, BS*R) clipped to the
-// colour.  Consecutive tiles cover the same spatial slab in every colour, and
-// consecutive blocks of an XCD take consecutive runs of `tpb` tiles, so the
-// x gathers of the rows in flight stay inside a compact slab.
-struct Tiles
-{
-    int ncolors;
-    int jtiles;   // max over colours of ceil(size / (BS*R))
-    int tpb;      // tiles per block
-    int off[MAX_COLORS];
-    int size[MAX_COLORS];
-};
 
 template <typename VT, typename XT, typename YT, bool DOT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restrict__ col,
@@ -245,25 +42,8 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
         const int64_t i = (t / tl.ncolors) * (BS * R) + (int64_t)threadIdx.x * R;
         if(i >= tl.size[c]) continue;
         const int64_t row = tl.off[c] + i;
-        int32_t cc[W][R];
-        VT av[W][R];
-#pragma unroll
-        for(int p = 0; p < W; ++p) ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
-#pragma unroll
-        for(int p = 0; p < W; ++p) ld_nt<R>(val + (int64_t)p * m + row, av[p]);
-        XT xv[W][R];
-#pragma unroll
-        for(int p = 0; p < W; ++p)
-#pragma unroll
-            for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
         XT s[R];
-#pragma unroll
-        for(int k = 0; k < R; ++k) s[k] = 0;
-#pragma unroll
-        for(int p = 0; p < W; ++p)
-#pragma unroll
-            for(int k = 0; k < R; ++k)
-                if(cc[p][k] >= 0) s[k] = fmaT(cvt<XT>(av[p][k]), xv[p][k], s[k]);
+        row_spmv<VT, XT, R>(m, row, col, val, x, s);
         st_cvt<R>(y + row, s);
         if(DOT)
         {
@@ -283,200 +63,43 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
     if(i >= size) return;                                                             \
     const int64_t row = off + i;
 
-// Zero-initial-guess forward sweep of colour c: only entries left of the
-// diagonal (earlier colours) are non-zero in x.
 template <typename VT, typename XT, typename RT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ dgi,
                                                  const RT* __restrict__ r, XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    int32_t dg[R];
-    ld<R>(dgi + row, dg);
-    int dmax = dg[0];
-#pragma unroll
-    for(int k = 1; k < R; ++k) dmax = dg[k] > dmax ? dg[k] : dmax;
-    int32_t cc[W][R];
-    VT av[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-    {
-        if(p < dmax)
-        {
-            ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
-            ld_nt<R>(val + (int64_t)p * m + row, av[p]);
-        }
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(!(p < dg[k])) cc[p][k] = -1;
-    }
-    XT d[R], s[R];
-#pragma unroll
-    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)dg[k] * m + row + k]);
-    ld_cvt<R>(r + row, s);
-    XT xv[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(cc[p][k] >= 0) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
-#pragma unroll
-    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
-    st<R>(x + row, s);
+    row_fwd_zero<VT, XT, RT, R>(m, row, col, val, dgi, r, x);
 }
 
-// Zero-initial-guess backward sweep of colour c: x = (x*d - U x) / d.
 template <typename VT, typename XT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_bwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ dgi,
                                                  XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    int32_t dg[R];
-    ld<R>(dgi + row, dg);
-    int dmin = dg[0];
-#pragma unroll
-    for(int k = 1; k < R; ++k) dmin = dg[k] < dmin ? dg[k] : dmin;
-    int32_t cc[W][R];
-    VT av[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-    {
-        if(p > dmin)
-        {
-            ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
-            ld_nt<R>(val + (int64_t)p * m + row, av[p]);
-        }
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(!(p > dg[k])) cc[p][k] = -1;
-    }
-    XT d[R], s[R];
-#pragma unroll
-    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)dg[k] * m + row + k]);
-    ld<R>(x + row, s);
-#pragma unroll
-    for(int k = 0; k < R; ++k) s[k] = s[k] * d[k];
-    XT xv[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(cc[p][k] >= 0) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
-#pragma unroll
-    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
-    st<R>(x + row, s);
+    row_bwd_zero<VT, XT, R>(m, row, col, val, dgi, x);
 }
 
-// Full forward sweep of colour c over the whole row, in ascending column
-// order; keeps t = r - L x (the sum up to the diagonal) for the backward
-// sweep, whose lower neighbours do not change in between.
 template <typename VT, typename XT, typename RT, bool STORE_T, int R, int BS>
 __global__ __launch_bounds__(BS) void k_fwd_full(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const RT* __restrict__ r,
                                                  XT* __restrict__ x, XT* __restrict__ t, const int* gate)
 {
     SWEEP_ROW();
-    int32_t cc[W][R];
-    VT av[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p) ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
-#pragma unroll
-    for(int p = 0; p < W; ++p) ld_nt<R>(val + (int64_t)p * m + row, av[p]);
-    XT xv[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            xv[p][k] = (cc[p][k] >= 0 && (int64_t)cc[p][k] != row + k) ? x[cc[p][k]] : (XT)0;
-    XT s[R], tt[R], d[R];
-    ld_cvt<R>(r + row, s);
-#pragma unroll
-    for(int k = 0; k < R; ++k)
-    {
-        tt[k] = 0;
-        d[k] = 1;
-    }
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-        {
-            if(cc[p][k] >= 0)
-            {
-                if((int64_t)cc[p][k] == row + k)
-                {
-                    tt[k] = s[k];
-                    d[k] = cvt<XT>(av[p][k]);
-                }
-                else
-                {
-                    s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
-                }
-            }
-        }
-#pragma unroll
-    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
-    st<R>(x + row, s);
-    if(STORE_T) st<R>(t + row, tt);
+    row_fwd_full<VT, XT, RT, STORE_T, R>(m, row, col, val, r, x, t);
 }
 
-// Full backward sweep of colour c using t: x = (t - U x) / d.
 template <typename VT, typename XT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_bwd_full(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ dgi,
                                                  const XT* __restrict__ t, XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    int32_t dg[R];
-    ld<R>(dgi + row, dg);
-    int dmin = dg[0];
-#pragma unroll
-    for(int k = 1; k < R; ++k) dmin = dg[k] < dmin ? dg[k] : dmin;
-    int32_t cc[W][R];
-    VT av[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-    {
-        if(p > dmin)
-        {
-            ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
-            ld_nt<R>(val + (int64_t)p * m + row, av[p]);
-        }
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(!(p > dg[k])) cc[p][k] = -1;
-    }
-    XT d[R], s[R];
-#pragma unroll
-    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)dg[k] * m + row + k]);
-    ld<R>(t + row, s);
-    XT xv[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(cc[p][k] >= 0) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
-#pragma unroll
-    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
-    st<R>(x + row, s);
+    row_bwd_full<VT, XT, R>(m, row, col, val, dgi, t, x);
 }
 #undef SWEEP_ROW
 
-// Fused fine residual + injection for coarse natural row k:
-// rc[f2c_coarse[k]] = r[f] - sum_p a[f,p] x[col], f = f2c_fine[k].
 template <typename VT, typename XT, typename RT, typename CT, int BS>
 __global__ __launch_bounds__(BS) void k_restrict(int64_t m, int64_t mc, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ f2c_fine,
@@ -486,21 +109,7 @@ __global__ __launch_bounds__(BS) void k_restrict(int64_t m, int64_t mc, const in
     if(gated(gate)) return;
     const int64_t k = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if(k >= mc) return;
-    const int64_t row = f2c_fine[k];
-    int32_t cc[W][1];
-    VT av[W][1];
-#pragma unroll
-    for(int p = 0; p < W; ++p) ld_nt<1>(col + (int64_t)p * m + row, cc[p]);
-#pragma unroll
-    for(int p = 0; p < W; ++p) ld_nt<1>(val + (int64_t)p * m + row, av[p]);
-    XT xv[W];
-#pragma unroll
-    for(int p = 0; p < W; ++p) xv[p] = (cc[p][0] >= 0) ? x[cc[p][0]] : (XT)0;
-    XT s = cvt<XT>(r[row]);
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-        if(cc[p][0] >= 0) s = fmaT(-cvt<XT>(av[p][0]), xv[p], s);
-    rc[f2c_coarse[k]] = (CT)s;
+    row_restrict<VT, XT, RT, CT>(m, k, col, val, f2c_fine, f2c_coarse, r, x, rc);
 }
 
 // Prolongation: xf[f2c_fine[k]] += xc[f2c_coarse[k]].

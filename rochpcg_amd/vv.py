@@ -112,3 +112,21 @@ def testcg(prob):
         b.upload(b0.host())
         prob.vec["x"].upload(np.zeros(m))
     return out
+
+
+def splitmix64(seed, i):
+    """Output i of the splitmix64 generator seeded with `seed` (as an int)."""
+    mask = (1 << 64) - 1
+    z = (seed + (i + 1) * 0x9E3779B97F4A7C15) & mask
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & mask
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & mask
+    return z ^ (z >> 31)
+
+
+def batched_dims(count=64, seed=7, lo=16, hi=96, step=8):
+    """Grid of each problem of the batched workload: nx, ny, nz drawn
+    independently from {lo, lo+step, ..., hi}; dimension d of problem k takes
+    splitmix64(seed) output 3k+d modulo the number of choices."""
+    choices = list(range(lo, hi + 1, step))
+    return [tuple(choices[splitmix64(seed, 3 * k + d) % len(choices)] for d in range(3))
+            for k in range(count)]
