@@ -58,5 +58,8 @@ def test_batched_graph_equals_eager(hip):
     for P in B.problems:
         P.vec["x"].upload(np.zeros(P.m))
     c = B.cg(max_iter=25, use_graph=True)
-    assert np.allclose(c["normr"], a["normr"], rtol=1e-10, atol=0)
+    # equal up to the grouping of dot partials; the 16^3 problem is already at
+    # rounding noise (1e-16 relative), where only an absolute bound is meaningful
+    diff = np.abs(c["normr"] - a["normr"])
+    assert np.all((diff <= 1e-10 * a["normr"]) | (diff <= 1e-15 * a["normr0"]))
     B.close()
