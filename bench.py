@@ -154,6 +154,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1234, help="splitmix64 seed of x* in the solve-to-1e-9 leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-problems leg")
     ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--shapes", default=None,
+                    help="apply launch shapes saved by --save-shapes instead of tuning")
+    ap.add_argument("--save-shapes", default=None, help="write the tuned launch shapes to this JSON file")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs (--gpus > 1) "
@@ -171,12 +174,20 @@ def main():
     flops = G.flops_per_iteration()
     K, W = args.steps, args.warmup
     tuned = {}
-    if not args.no_tune:
+    if args.shapes or not args.no_tune:
         from rochpcg_amd import autotune
         t0 = time.perf_counter()
-        table = autotune.tune(G, mixed=False, log=log)
-        table = autotune.tune(G, mixed=True, table=table, log=log)
-        log(f"autotune: {time.perf_counter() - t0:.1f} s")
+        if args.shapes:
+            table = autotune.load(args.shapes)
+            if not autotune.apply(G, table):
+                sys.exit(f"bench: no launch shapes for this problem in {args.shapes}")
+            log(f"launch shapes from {args.shapes}")
+        else:
+            table = autotune.tune(G, mixed=False, log=log)
+            table = autotune.tune(G, mixed=True, table=table, log=log)
+            log(f"autotune: {time.perf_counter() - t0:.1f} s")
+        if args.save_shapes:
+            autotune.save(table, args.save_shapes)
         L0 = G.levels[0]
         tuned = {k: {op: v["shape"] for op, v in e.items()} for k, e in table.items()
                  if k.startswith(f"{L0.nx}x{L0.ny}x{L0.nz}:")}

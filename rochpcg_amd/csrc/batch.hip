@@ -35,32 +35,14 @@ __device__ __forceinline__ int find_seg(const BSeg* __restrict__ segs, int nseg,
     return lo;
 }
 
-// block partials of one segment; the segment's last block folds them
+// the fold of a reducing launch: block k folds segment k's partials (one per
+// block of the segment) for that segment's problem
 template <int BS>
-__device__ __forceinline__ void reduce_finish_seg(double v, const RedArgs& ra, int64_t lb, int64_t nblk)
+__global__ __launch_bounds__(BS) void kb_fold(const BSeg* __restrict__ segs)
 {
-    __shared__ double lds[BS / 64];
-    __shared__ int is_last;
-    const double bsum = block_sum<BS>(v, lds);
-    if(threadIdx.x == 0)
-    {
-        ra.partial[lb] = bsum;
-        __threadfence();
-        const unsigned t = atomicAdd(ra.ticket, 1u);
-        is_last = (t == (unsigned)(nblk - 1)) ? 1 : 0;
-    }
-    __syncthreads();
-    if(!is_last) return;
-    __threadfence();
-    double s = 0.0;
-    for(int64_t i = threadIdx.x; i < nblk; i += BS) s += __builtin_nontemporal_load(ra.partial + i);
-    __syncthreads();
-    s = block_sum<BS>(s, lds);
-    if(threadIdx.x == 0)
-    {
-        apply_op(ra.op, s, ra.sc, ra.hist, ra.hist_cap, ra.out);
-        atomicExch(ra.ticket, 0u);
-    }
+    const BSeg& S = segs[blockIdx.x];
+    if(gated(S.gate) || S.ra.partial == nullptr) return;
+    fold_partials<BS>(S.ra.partial, S.nblk, S.ra);
 }
 
 #define BSEG()                                                \
@@ -83,7 +65,7 @@ __global__ __launch_bounds__(BS) void kb_spmv(const BSeg* __restrict__ segs, int
         ((YT*)S.x)[row] = (YT)s[0];
         if(DOT) acc = fmaT((double)((const XT*)S.r)[row], (double)s[0], acc);
     }
-    if(DOT) reduce_finish_seg<BS>(acc, S.ra, lb, S.nblk);
+    if(DOT) block_partial<BS>(acc, S.ra.partial, lb);
 }
 
 template <typename VT, typename XT, typename RT, int BS>
@@ -163,7 +145,7 @@ __global__ __launch_bounds__(BS) void kb_dot(const BSeg* __restrict__ segs, int 
     const YT* bb = (const YT*)S.b;
     double acc = 0.0;
     for(int64_t i = e0 + threadIdx.x; i < e1; i += BS) acc = fmaT((double)bb[i], (double)a[i], acc);
-    reduce_finish_seg<BS>(acc, S.ra, lb, S.nblk);
+    block_partial<BS>(acc, S.ra.partial, lb);
 }
 
 template <typename ZT, int BS>
@@ -196,7 +178,7 @@ __global__ __launch_bounds__(BS) void kb_cg_xr(const BSeg* __restrict__ segs, in
         x[i] = fmaT(alpha, p[i], x[i]);
         acc = fmaT(rv, rv, acc);
     }
-    reduce_finish_seg<BS>(acc, S.ra, lb, S.nblk);
+    block_partial<BS>(acc, S.ra.partial, lb);
 }
 
 template <int BS>
@@ -322,6 +304,11 @@ static int launch(const BLaunch& L, hipStream_t s)
     default: launch_bs<256>(L, s); break;
     }
     HPCGX_HIP(hipGetLastError());
+    if(L.kind == BK_DOT || L.kind == BK_CG_XR || (L.kind == BK_SPMV && L.dot))
+    {
+        hipLaunchKernelGGL((kb_fold<FOLD_THREADS>), dim3((unsigned)L.nseg), dim3(FOLD_THREADS), 0, s, L.segs);
+        HPCGX_HIP(hipGetLastError());
+    }
     return 0;
 }
 
@@ -393,7 +380,6 @@ struct Planner
     {
         RedArgs ra;
         ra.partial = P->partial;
-        ra.ticket = P->ticket;
         ra.op = op;
         ra.sc = P->sc;
         ra.hist = P->hist;

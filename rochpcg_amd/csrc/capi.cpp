@@ -233,7 +233,6 @@ static RedArgs store_red(Problem* P)
 {
     RedArgs ra;
     ra.partial = P->partial;
-    ra.ticket = P->ticket;
     ra.op = OP_STORE;
     ra.out = P->dout;
     ra.cap = P->partial_cap;

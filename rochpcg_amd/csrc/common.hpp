@@ -119,15 +119,12 @@ struct CGScalars
     int pad;
 };
 
-// Scratch for deterministic two-pass reductions: per-block partials plus a
-// ticket counter; the last block to finish folds the partials in index order.
-constexpr int RED_BLOCKS = 1024;
+// Deterministic two-phase reductions: every block of the producing kernel
+// writes one partial, then a one-block fold kernel sums them in a fixed order.
+constexpr int RED_BLOCKS = 4096;    // blocks of a streaming vector reduction
 constexpr int RED_THREADS = 256;
-struct Reduce
-{
-    double* partial = nullptr;      // [RED_BLOCKS * 2]
-    unsigned int* ticket = nullptr; // [4]
-};
+constexpr int FOLD_MAX = 16384;     // partials a single-problem fold reads at most
+constexpr int FOLD_THREADS = 1024;
 
 // ---------------------------------------------------------------------------
 // device helpers

@@ -28,7 +28,6 @@ struct Context
 {
     hipStream_t s = nullptr;
     double* partial = nullptr;
-    unsigned* ticket = nullptr;
     double* dout = nullptr;
     double h_out = 0.0;
     bool ok = false;
@@ -40,10 +39,8 @@ Context& ctx()
     if(!c.ok)
     {
         if(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking) != hipSuccess ||
-           hipMalloc(&c.partial, sizeof(double) * RED_BLOCKS) != hipSuccess ||
-           hipMalloc(&c.ticket, 4 * sizeof(unsigned)) != hipSuccess ||
-           hipMalloc(&c.dout, 8 * sizeof(double)) != hipSuccess ||
-           hipMemsetAsync(c.ticket, 0, 4 * sizeof(unsigned), c.s) != hipSuccess)
+           hipMalloc(&c.partial, sizeof(double) * FOLD_MAX) != hipSuccess ||
+           hipMalloc(&c.dout, 8 * sizeof(double)) != hipSuccess)
         {
             fprintf(stderr, "rochpcg_amd: cannot initialise the HIP context\n");
             return c;
@@ -76,10 +73,9 @@ RedArgs store_red()
 {
     RedArgs ra;
     ra.partial = ctx().partial;
-    ra.ticket = ctx().ticket;
     ra.op = OP_STORE;
     ra.out = ctx().dout;
-    ra.cap = RED_BLOCKS;
+    ra.cap = FOLD_MAX;
     return ra;
 }
 

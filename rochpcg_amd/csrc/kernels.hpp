@@ -25,8 +25,8 @@ __device__ __forceinline__ T cvt(__half v) { return (T)__half2float(v); }
 __device__ __forceinline__ bool gated(const int* gate) { return gate != nullptr && *gate != 0; }
 
 // ---------------------------------------------------------------------------
-// deterministic block-partials reduction; the last block folds the partials
-// in index order, applies `op` and re-arms the ticket.
+// deterministic two-phase reductions (block partials, then one fold block);
+// apply_op is what the fold does with the total.
 __device__ inline void apply_op(int op, double v, CGScalars* sc, double* hist, int hist_cap, double* out)
 {
     switch(op)
@@ -74,31 +74,35 @@ __device__ inline void apply_op(int op, double v, CGScalars* sc, double* hist, i
     }
 }
 
+// Phase 1: the block sum of v goes to partial[slot].
 template <int BS>
-__device__ __forceinline__ void reduce_finish(double v, const RedArgs& ra)
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ partial, int64_t slot)
 {
     __shared__ double lds[BS / 64];
-    __shared__ int is_last;
     const double bsum = block_sum<BS>(v, lds);
-    if(threadIdx.x == 0)
+    if(threadIdx.x == 0) partial[slot] = bsum;
+}
+
+// Phase 2, run by one block of BS threads in the kernel launched after the
+// producer on the same stream (the kernel boundary orders the two): thread j
+// sums partials j, j+BS, ... in four interleaved chains, the block sums the
+// threads, thread 0 applies `op`.  The grouping depends only on n.
+template <int BS>
+__device__ __forceinline__ void fold_partials(const double* __restrict__ partial, int64_t n, const RedArgs& ra)
+{
+    __shared__ double lds[BS / 64];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t i = threadIdx.x;
+    for(; i + 3 * BS < n; i += 4 * BS)
     {
-        ra.partial[blockIdx.x] = bsum;
-        __threadfence();
-        const unsigned t = atomicAdd(ra.ticket, 1u);
-        is_last = (t == gridDim.x - 1) ? 1 : 0;
+        a0 += partial[i];
+        a1 += partial[i + BS];
+        a2 += partial[i + 2 * BS];
+        a3 += partial[i + 3 * BS];
     }
-    __syncthreads();
-    if(!is_last) return;
-    __threadfence();
-    double s = 0.0;
-    for(unsigned i = threadIdx.x; i < gridDim.x; i += BS) s += __builtin_nontemporal_load(ra.partial + i);
-    __syncthreads();
-    s = block_sum<BS>(s, lds);
-    if(threadIdx.x == 0)
-    {
-        apply_op(ra.op, s, ra.sc, ra.hist, ra.hist_cap, ra.out);
-        atomicExch(ra.ticket, 0u);
-    }
+    for(; i < n; i += BS) a0 += partial[i];
+    const double s = block_sum<BS>((a0 + a1) + (a2 + a3), lds);
+    if(threadIdx.x == 0) apply_op(ra.op, s, ra.sc, ra.hist, ra.hist_cap, ra.out);
 }
 
 // ---------------------------------------------------------------------------

@@ -20,6 +20,7 @@
 #include "kernels.hpp"
 #include "ops.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace hpcgx
@@ -53,13 +54,22 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
             for(int k = 0; k < R; ++k) acc = fmaT((double)xr[k], (double)s[k], acc);
         }
     }
-    if(DOT) reduce_finish<BS>(acc, ra);
+    if(DOT) block_partial<BS>(acc, ra.partial, blockIdx.x);
 }
 
-// Per-colour sweeps: block b covers rows off + xcd_block(b)*BS*R + [0, BS*R).
+// Per-colour sweeps: block b covers rows off + L*BS*R + [0, BS*R) with
+// L = xcd_block(b).  Consecutive colour launches alternate direction (size < 0
+// marks a reversed launch): each XCD then starts a launch on the slab it
+// finished the previous one on, whose neighbours' x is still in its caches.
 #define SWEEP_ROW()                                                                   \
     if(gated(gate)) return;                                                           \
-    const int64_t i = xcd_block(blockIdx.x, gridDim.x) * (BS * R) + (int64_t)threadIdx.x * R; \
+    int64_t Lb_ = xcd_block(blockIdx.x, gridDim.x);                                   \
+    if(size < 0)                                                                      \
+    {                                                                                 \
+        size = -size;                                                                 \
+        Lb_ = (int64_t)gridDim.x - 1 - Lb_;                                           \
+    }                                                                                 \
+    const int64_t i = Lb_ * (BS * R) + (int64_t)threadIdx.x * R;                      \
     if(i >= size) return;                                                             \
     const int64_t row = off + i;
 
@@ -158,6 +168,14 @@ __global__ __launch_bounds__(256) void k_restrict_diff(int64_t mc, const int32_t
 // ---------------------------------------------------------------------------
 // CG vector kernels (f64 vectors; z may be f32 in the mixed mode).  Chunked:
 // block L (XCD-contiguous) handles elements [L*chunk, (L+1)*chunk).
+// second phase of every reduction: one block folds the producer's n partials
+template <int BS>
+__global__ __launch_bounds__(BS) void k_fold(int64_t n, RedArgs ra, const int* gate)
+{
+    if(gated(gate)) return;
+    fold_partials<BS>(ra.partial, n, ra);
+}
+
 __device__ __forceinline__ void chunk_range(int64_t n, int64_t& b, int64_t& e)
 {
     const int64_t L = xcd_block(blockIdx.x, gridDim.x);
@@ -175,7 +193,7 @@ __global__ __launch_bounds__(BS) void k_dot(int64_t n, const XT* __restrict__ x,
     chunk_range(n, b, e);
     double acc = 0.0;
     for(int64_t i = b + threadIdx.x; i < e; i += BS) acc = fmaT((double)y[i], (double)x[i], acc);
-    reduce_finish<BS>(acc, ra);
+    block_partial<BS>(acc, ra.partial, blockIdx.x);
 }
 
 // w = alpha*x + beta*y with the reference's special cases (ComputeWAXPBY.cpp:63-91)
@@ -212,7 +230,7 @@ __global__ __launch_bounds__(BS) void k_axpy_dot(int64_t n, double alpha, const 
         y[i] = v;
         acc = fmaT(v, v, acc);
     }
-    reduce_finish<BS>(acc, ra);
+    block_partial<BS>(acc, ra.partial, blockIdx.x);
 }
 
 // p = z (first iteration) or p = z + beta*p, with beta read on the device
@@ -247,7 +265,7 @@ __global__ __launch_bounds__(BS) void k_cg_xr(int64_t n, const double* __restric
         x[i] = fmaT(alpha, p[i], x[i]);
         acc = fmaT(rv, rv, acc);
     }
-    reduce_finish<BS>(acc, ra);
+    block_partial<BS>(acc, ra.partial, blockIdx.x);
 }
 
 template <typename T, int BS>
@@ -302,6 +320,14 @@ static inline unsigned nblk(int64_t n, int bs) { return (unsigned)ceil_div(n, bs
 
 #define HPCGX_KCHECK() HPCGX_HIP(hipGetLastError())
 
+// the fold that completes a reduction whose producer ran `nparts` blocks
+int launch_fold(int64_t nparts, const RedArgs& ra, hipStream_t s, const int* gate)
+{
+    hipLaunchKernelGGL((k_fold<FOLD_THREADS>), dim3(1), dim3(FOLD_THREADS), 0, s, nparts, ra, gate);
+    HPCGX_KCHECK();
+    return 0;
+}
+
 // Launch shapes compiled in: rows per thread {1,2,4} x block {64,128,256}.
 template <typename F>
 static int with_shape(const Shape& sh, F&& f)
@@ -351,10 +377,10 @@ static Tiles make_tiles(const Level& L, int rows_per_tile, bool dot, int64_t cap
     }
     t.jtiles = (int)ceil_div(mx, rows_per_tile);
     const int64_t ntiles = (int64_t)t.ncolors * t.jtiles;
-    // one slab of every colour per block keeps the partials few and the
-    // blocks of an XCD on neighbouring slabs
-    t.tpb = dot ? t.ncolors : 1;
-    while(dot && ceil_div(ntiles, t.tpb) > cap) t.tpb += t.ncolors;
+    // with a fused dot, a block walks tpb consecutive tiles (neighbouring
+    // slabs of the colours) so the partials stay few enough for one fold
+    t.tpb = 1;
+    while(dot && ceil_div(ntiles, t.tpb) > cap) ++t.tpb;
     nblocks = ceil_div(ntiles, t.tpb);
     return t;
 }
@@ -374,10 +400,14 @@ static int spmv_t(const Level& L, const void* x, void* y, const RedArgs* ra, hip
     return with_shape(sh, [&](auto RR, auto BB) -> int {
         constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
         int64_t nb;
-        Tiles tl = make_tiles(L, BS * R, ra != nullptr, ra ? ra->cap : 0, nb);
+        Tiles tl = make_tiles(L, BS * R, ra != nullptr, ra ? std::min<int64_t>(ra->cap, FOLD_MAX) : 0, nb);
         if(ra)
+        {
             hipLaunchKernelGGL((k_spmv<VT, XT, YT, true, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m, L.ell_col,
                                vals_of<VT>(L), (const XT*)x, (YT*)y, tl, *ra, gate);
+            HPCGX_KCHECK();
+            return launch_fold(nb, *ra, s, gate);
+        }
         else
             hipLaunchKernelGGL((k_spmv<VT, XT, YT, false, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m,
                                L.ell_col, vals_of<VT>(L), (const XT*)x, (YT*)y, tl, RedArgs{}, gate);
@@ -411,7 +441,11 @@ static int per_color(const Level& L, int prec, int which, int c_first, int c_las
         {
             if(L.sizes[c] == 0) continue;
             const dim3 grid(nblk(ceil_div(L.sizes[c], R), BS));
-            launch(std::integral_constant<int, R>{}, std::integral_constant<int, BS>{}, grid, c);
+            // odd colours walk their blocks backwards (in both sweeps: forward
+            // ends on colour 7 reversed, backward starts on colour 6 forwards,
+            // so every launch starts on the slab the previous one ended on)
+            const int64_t size = (c & 1) ? -(int64_t)L.sizes[c] : (int64_t)L.sizes[c];
+            launch(std::integral_constant<int, R>{}, std::integral_constant<int, BS>{}, grid, c, size);
             HPCGX_KCHECK();
         }
         return 0;
@@ -433,34 +467,34 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
     switch(part)
     {
     case 0:
-        return per_color(L, pr, SH_FWD_ZERO, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c) {
+        return per_color(L, pr, SH_FWD_ZERO, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             hipLaunchKernelGGL((k_fwd_zero<VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
-                               (int64_t)L.sizes[c], L.ell_col, val, L.diag_idx, r, x, gate);
+                               sz, L.ell_col, val, L.diag_idx, r, x, gate);
         });
     case 1:
         if(last < 1) return 0;
-        return per_color(L, pr, SH_BWD_ZERO, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c) {
+        return per_color(L, pr, SH_BWD_ZERO, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             hipLaunchKernelGGL((k_bwd_zero<VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
-                               (int64_t)L.sizes[c], L.ell_col, val, L.diag_idx, x, gate);
+                               sz, L.ell_col, val, L.diag_idx, x, gate);
         });
     case 2:
-        return per_color(L, pr, SH_FWD_FULL, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c) {
+        return per_color(L, pr, SH_FWD_FULL, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             if(c < last)
                 hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], (int64_t)L.sizes[c], L.ell_col, val, r, x, t, gate);
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, r, x, t, gate);
             else
                 hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], (int64_t)L.sizes[c], L.ell_col, val, r, x, t, gate);
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, r, x, t, gate);
         });
     case 3:
         if(last < 1) return 0;
-        return per_color(L, pr, SH_BWD_FULL, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c) {
+        return per_color(L, pr, SH_BWD_FULL, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             hipLaunchKernelGGL((k_bwd_full<VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
-                               (int64_t)L.sizes[c], L.ell_col, val, L.diag_idx, (const XT*)t, x, gate);
+                               sz, L.ell_col, val, L.diag_idx, (const XT*)t, x, gate);
         });
     }
     set_error("invalid sweep part");
@@ -573,7 +607,7 @@ int launch_dot(int64_t n, const void* x, bool x_f64, const void* y, bool y_f64, 
         hipLaunchKernelGGL((k_dot<float, float, BS>), dim3(g), dim3(BS), 0, s, n, (const float*)x, (const float*)y,
                            ra, gate);
     HPCGX_KCHECK();
-    return 0;
+    return launch_fold(g, ra, s, gate);
 }
 
 int launch_waxpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* w, hipStream_t s,
@@ -589,9 +623,10 @@ int launch_axpy_dot(int64_t n, double alpha, const double* x, double* y, const R
                     const int* gate)
 {
     constexpr int BS = RED_THREADS;
-    hipLaunchKernelGGL((k_axpy_dot<BS>), dim3(red_grid(n)), dim3(BS), 0, s, n, alpha, x, y, ra, gate);
+    const unsigned g = red_grid(n);
+    hipLaunchKernelGGL((k_axpy_dot<BS>), dim3(g), dim3(BS), 0, s, n, alpha, x, y, ra, gate);
     HPCGX_KCHECK();
-    return 0;
+    return launch_fold(g, ra, s, gate);
 }
 
 int launch_cg_p(int64_t n, const void* z, bool z_f64, double* p, const CGScalars* sc, hipStream_t s, const int* gate)
@@ -609,9 +644,10 @@ int launch_cg_xr(int64_t n, const double* p, const double* Ap, double* x, double
                  hipStream_t s, const int* gate)
 {
     constexpr int BS = RED_THREADS;
-    hipLaunchKernelGGL((k_cg_xr<BS>), dim3(red_grid(n)), dim3(BS), 0, s, n, p, Ap, x, r, ra, gate);
+    const unsigned g = red_grid(n);
+    hipLaunchKernelGGL((k_cg_xr<BS>), dim3(g), dim3(BS), 0, s, n, p, Ap, x, r, ra, gate);
     HPCGX_KCHECK();
-    return 0;
+    return launch_fold(g, ra, s, gate);
 }
 
 int launch_fill(int64_t n, void* p, bool f64, double v, hipStream_t s, const int* gate)

@@ -19,8 +19,7 @@ enum RedOp : int
 
 struct RedArgs
 {
-    double* partial = nullptr;      // >= RED_BLOCKS doubles
-    unsigned int* ticket = nullptr; // zero between reductions
+    double* partial = nullptr;      // one slot per block of the producing kernel
     int op = OP_NONE;
     CGScalars* sc = nullptr;
     double* hist = nullptr;
@@ -30,6 +29,9 @@ struct RedArgs
 };
 
 bool shape_supported(int rows, int block);
+
+// one-block fold of `nparts` partials written by the previous launch on `s`
+int launch_fold(int64_t nparts, const RedArgs& ra, hipStream_t s, const int* gate);
 
 // `gate` (device int, may be null): kernels return at once while *gate != 0.
 int launch_spmv(const Level& L, Prec prec, const void* x, void* y, hipStream_t s, const int* gate);

@@ -103,14 +103,13 @@ int problem_create(Problem*& out, int nx, int ny, int nz, int nlevels, bool with
     if(with_f64 && alloc((void**)&P->z64, m0 * 8)) return fail(-1);
     if(with_mixed && alloc((void**)&P->z32, m0 * 4)) return fail(-1);
     P->hist_cap = 4096;
-    // fused SpMV + dot keeps one partial per slab of 64 rows at most
-    P->partial_cap = RED_BLOCKS + (int64_t)(m0 / 64) + 64;
+    // a batched fused SpMV + dot keeps one partial per 64-row block
+    P->partial_cap = std::max<int64_t>(FOLD_MAX, (int64_t)(m0 / 64) + 64);
     if(alloc((void**)&P->sc, sizeof(CGScalars)) || alloc((void**)&P->hist, sizeof(double) * (P->hist_cap + 1)) ||
-       alloc((void**)&P->partial, sizeof(double) * P->partial_cap) || alloc((void**)&P->ticket, 4 * sizeof(unsigned)) ||
+       alloc((void**)&P->partial, sizeof(double) * P->partial_cap) ||
        alloc((void**)&P->dout, 8 * sizeof(double)))
         return fail(-1);
-    if(hipMemsetAsync(P->ticket, 0, 4 * sizeof(unsigned), P->s) != hipSuccess ||
-       hipMemsetAsync(P->sc, 0, sizeof(CGScalars), P->s) != hipSuccess)
+    if(hipMemsetAsync(P->sc, 0, sizeof(CGScalars), P->s) != hipSuccess)
         return fail(-1);
     if(make_rhs(P->L[0], 0, 0, P->b, P->xexact, P->x, P->s)) return fail(-1);
     if(hipStreamSynchronize(P->s) != hipSuccess)
@@ -159,7 +158,6 @@ void problem_destroy(Problem* P)
     f(P->sc);
     f(P->hist);
     f(P->partial);
-    f(P->ticket);
     f(P->dout);
     if(P->own_stream && P->s) (void)hipStreamDestroy(P->s);
     delete P;
@@ -215,7 +213,6 @@ static RedArgs red(Problem& P, int op)
 {
     RedArgs ra;
     ra.partial = P.partial;
-    ra.ticket = P.ticket;
     ra.op = op;
     ra.sc = P.sc;
     ra.hist = P.hist;

@@ -49,7 +49,6 @@ struct Problem
     int hist_cap = 0;
     double* partial = nullptr;
     int64_t partial_cap = 0;   // reduction partial slots
-    unsigned int* ticket = nullptr;
     double* dout = nullptr;   // OP_STORE results
     bool ext_cg = false;      // r/z64/p/Ap belong to the caller (drop-in CGData)
     double h_out[8] = {};     // host landing slot of OP_STORE results
