@@ -63,10 +63,13 @@ int hpcgx_level_info(const hpcgx_problem* p, int level, int64_t* info);
 int hpcgx_color_table(const hpcgx_problem* p, int level, int* sizes, int* offsets, int* round_sizes);
 /* device pointers of a level's arrays (SparseMatrix::ell_col_ind, ell_val,
  * diag_idx, inv_diag, perm; MGData::d_f2cOperator); any output may be NULL.
- * ptrs[0..8] = ell_col_ind, ell_val (f64 or NULL), ell_val_f16 (or NULL),
- * diag_idx, inv_diag, perm (natural -> multicolour), colors (natural),
- * f2c (coarse natural -> fine natural), f2c_fine (coarse natural -> fine
- * multicolour row), f2c_coarse (coarse natural -> coarse multicolour row) */
+ * ptrs must hold 11 entries: ptrs[0..10] = ell_col_ind, ell_val (f64 or
+ * NULL), ell_val_f16 (or NULL), diag_idx, inv_diag, perm (natural ->
+ * multicolour), colors (natural), f2c (coarse natural -> fine natural),
+ * f2c_fine (coarse natural -> fine multicolour row), f2c_coarse (coarse
+ * natural -> coarse multicolour row), row descriptors (int32: diagonal slot
+ * in bits 0-7, first slot of a last-colour column in bits 8-15, valid slots
+ * in bits 16-23; no reference counterpart) */
 int hpcgx_level_arrays(const hpcgx_problem* p, int level, void** ptrs);
 /* device pointers of the CG data and system: ptrs[0..7] = b, x, xexact, r,
  * z (f64, or NULL), p, Ap, z (f32 mixed-mode, or NULL)  (CGData.hpp:55-61) */
@@ -167,9 +170,10 @@ int hpcgx_batch_cg(hpcgx_batch* b, int mode, int max_iter, double tolerance, int
  * `level`, measured with HIP events on the problem's stream, with the
  * operands the V-cycle uses there (level 0: the CG residual r and z).
  * op 0 = SpMV, 1 = SymGS, 2 = zero-guess SymGS, 3 = MG from `level`,
- * 4/5 = zero-guess forward/backward sweeps, 6/7 = forward/backward sweeps,
- * 8 = fused residual + restriction; prec 0 = f64 store, 1 = mixed store
- * (SpMV: f16 values with f64 vectors). */
+ * 4/5 = zero-guess forward/backward sweeps, 6/7 = forward/backward sweeps
+ * (in the form the V-cycle runs them on that level), 8 = fused residual +
+ * restriction; prec 0 = f64 store, 1 = mixed store (SpMV: f16 values with
+ * f64 vectors). */
 int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps, double* avg_ms);
 /* launch shape of an operator on a level (the autotuner's knobs; no
  * reference counterpart -- the reference hard-codes 1024/256/128-thread

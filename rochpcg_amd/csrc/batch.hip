@@ -74,7 +74,7 @@ __global__ __launch_bounds__(BS) void kb_fwd_zero(const BSeg* __restrict__ segs,
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    row_fwd_zero<VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.dg, (const RT*)S.r, (XT*)S.x);
+    row_fwd_zero<VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x);
 }
 
 template <typename VT, typename XT, int BS>
@@ -83,7 +83,10 @@ __global__ __launch_bounds__(BS) void kb_bwd_zero(const BSeg* __restrict__ segs,
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    row_bwd_zero<VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.dg, (XT*)S.x);
+    if(S.t)
+        row_bwd_zero<VT, XT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, (XT*)S.t);
+    else
+        row_bwd_zero<VT, XT, false, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, nullptr);
 }
 
 template <typename VT, typename XT, typename RT, int BS>
@@ -92,7 +95,10 @@ __global__ __launch_bounds__(BS) void kb_fwd_full(const BSeg* __restrict__ segs,
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    if(S.t)
+    if(S.use_u)
+        row_fwd_full_u<VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x,
+                                      (XT*)S.t);
+    else if(S.t)
         row_fwd_full<VT, XT, RT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
                                           (XT*)S.t);
     else
@@ -106,7 +112,7 @@ __global__ __launch_bounds__(BS) void kb_bwd_full(const BSeg* __restrict__ segs,
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    row_bwd_full<VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.dg, (const XT*)S.t, (XT*)S.x);
+    row_bwd_full<VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const XT*)S.t, (XT*)S.x);
 }
 
 template <typename VT, typename XT, typename RT, int BS>
@@ -415,7 +421,7 @@ struct Planner
             s.size = L.sizes[c];
             s.col = L.ell_col;
             s.val = vals(L);
-            s.dg = L.diag_idx;
+            s.desc = L.desc;
             s.r = (l == 0) ? (const void*)P->r : vecs(P)[l].r;
             s.x = (l == 0) ? z(P) : vecs(P)[l].x;
             return true;
@@ -423,9 +429,13 @@ struct Planner
         const int C = max_colors(l);
         for(int c = 0; c < C; ++c)
             HPCGX_TRY(add(BK_FWD_ZERO, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) { return level_seg(k, s, c); }));
+        // the pre-smoother keeps its upper sums for the post-smoother where
+        // the correction changes only last-colour rows (Level::u_ok)
         for(int c = C - 2; c >= 0; --c)
             HPCGX_TRY(add(BK_BWD_ZERO, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
-                return c <= B.probs[k]->L[l].ncolors - 2 && level_seg(k, s, c);
+                if(!(c <= B.probs[k]->L[l].ncolors - 2 && level_seg(k, s, c))) return false;
+                if(!last && B.probs[k]->L[l].u_ok) s.t = vecs(B.probs[k])[l].t;
+                return true;
             }));
         if(last) return 0;
         HPCGX_TRY(add(BK_RESTRICT, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
@@ -456,7 +466,9 @@ struct Planner
         for(int c = 0; c < C; ++c)
             HPCGX_TRY(add(BK_FWD_FULL, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
                 if(!level_seg(k, s, c)) return false;
-                s.t = (c < B.probs[k]->L[l].ncolors - 1) ? vecs(B.probs[k])[l].t : nullptr;
+                const bool before_last = c < B.probs[k]->L[l].ncolors - 1;
+                s.t = before_last ? vecs(B.probs[k])[l].t : nullptr;
+                s.use_u = before_last && B.probs[k]->L[l].u_ok;
                 return true;
             }));
         for(int c = C - 2; c >= 0; --c)

@@ -19,10 +19,12 @@ struct BSeg
     int64_t size;      // rows (or elements) in the segment
     const int32_t* col;
     const void* val;
-    const int32_t* dg;
+    const int32_t* desc;   // row descriptors (Level::desc)
     const void* r;     // rhs / first vector operand
     void* x;           // solution / in-out operand
-    void* t;           // SymGS partial sums (null: do not keep them)
+    void* t;           // SymGS partial sums (null: do not keep them); upper sums in a
+                       // pre-smoother backward sweep (null: not kept)
+    int use_u;         // forward sweep reuses the upper sums in t (row_fwd_full_u)
     const int32_t* f2c_fine;
     const int32_t* f2c_coarse;
     void* rc;          // coarse vector (restriction output / prolongation input)

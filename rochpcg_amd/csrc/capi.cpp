@@ -128,8 +128,8 @@ int hpcgx_level_arrays(const hpcgx_problem* p, int level, void** ptrs)
     CHECK_LEVEL(p, level);
     const Level& L = p->P->L[level];
     void* v[] = {L.ell_col, L.ell_val, L.ell_h, L.diag_idx, L.inv_diag, L.perm, L.colors, L.f2c_nat, L.f2c_fine,
-                 L.f2c_coarse};
-    for(int i = 0; i < 10; ++i) ptrs[i] = v[i];
+                 L.f2c_coarse, L.desc};
+    for(int i = 0; i < 11; ++i) ptrs[i] = v[i];
     return 0;
 }
 
@@ -377,7 +377,12 @@ int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps,
         case 2: return launch_symgs_zero(L, pr, r, r_f64, x, P->s, nullptr);
         case 3: return mg(*P, level, pr, r, r_f64, x, nullptr);
         case 4: case 5: case 6: case 7:
-            return launch_sweep(L, pr, op - 4, r, r_f64, x, V[level].t, P->s, nullptr);
+        {
+            // the variants the V-cycle runs on this level
+            int part = op - 4;
+            if(L.u_ok && level + 1 < P->nlevels && (part == 1 || part == 2)) part += 3;
+            return launch_sweep(L, pr, part, r, r_f64, x, V[level].t, P->s, nullptr);
+        }
         case 8: return launch_restrict(L, pr, r, r_f64, x, V[level + 1].r, P->s, nullptr);
         }
         return -2;

@@ -86,6 +86,7 @@ struct Level
     double* ell_val = nullptr;    // [W*m] (f64 levels)
     __half* ell_h = nullptr;      // [W*m] (mixed levels)
     int32_t* diag_idx = nullptr;  // [m] slot of the diagonal
+    int32_t* desc = nullptr;      // [m] row descriptor: diagonal slot, first last-colour slot, valid slots
     double* inv_diag = nullptr;   // [m] 1/a_ii (f64)
     float* diag_f = nullptr;      // [m] a_ii in f32 (mixed levels)
     int32_t* perm = nullptr;      // [m] natural row -> multicolour row
@@ -97,6 +98,10 @@ struct Level
     int32_t* f2c_fine = nullptr;    // [mc] multicolour fine row injected into k
     int32_t* f2c_coarse = nullptr;  // [mc] multicolour coarse row of k
     int32_t* f2c_nat = nullptr;     // [mc] natural fine row (reference f2cOperator)
+    // every injected fine row is in the last colour: the prolongation changes
+    // only last-colour rows, so the post-smoother may reuse the pre-smoother's
+    // upper sums over the other colours
+    bool u_ok = false;
 
     Shape shp[2][SH_COUNT];   // [precision][operator]
 };

@@ -247,95 +247,129 @@ __device__ __forceinline__ void row_spmv(int64_t m, int64_t row, const int32_t* 
             if(cc[p][k] >= 0) s[k] = fmaT(cvt<XT>(av[p][k]), xv[p][k], s[k]);
 }
 
-// Zero-initial-guess forward sweep: only entries left of the diagonal
-// (earlier colours) are non-zero in x.
-template <typename VT, typename XT, typename RT, int R>
-__device__ __forceinline__ void row_fwd_zero(int64_t m, int64_t row, const int32_t* __restrict__ col,
-                                             const VT* __restrict__ val, const int32_t* __restrict__ dgi,
-                                             const RT* __restrict__ r, XT* __restrict__ x)
+// Row descriptor, one int32 per row built with the ELL (slots hold the
+// columns in ascending multicolour order, padding last): bits 0-7 the slot of
+// the diagonal d, bits 8-15 the first slot whose column lies in the last
+// colour tl (the end when there is none), bits 16-23 the number of valid
+// slots e.  So [0, d) are the lower entries, (d, tl) the upper entries of the
+// colours before the last and [tl, e) the entries of the last colour.
+__device__ __forceinline__ int desc_diag(int32_t v) { return v & 255; }
+__device__ __forceinline__ int desc_tail(int32_t v) { return (v >> 8) & 255; }
+__device__ __forceinline__ int desc_end(int32_t v) { return (v >> 16) & 255; }
+
+// Loads slot p of the thread's R rows where in(p, k) holds for some k, and
+// marks the slots of row k outside in(p, k) with column -1.
+template <typename VT, int R, typename In>
+__device__ __forceinline__ void load_slots(int64_t m, int64_t row, const int32_t* __restrict__ col,
+                                           const VT* __restrict__ val, int32_t (&cc)[W][R], VT (&av)[W][R], In in)
 {
-    int32_t dg[R];
-    ld<R>(dgi + row, dg);
-    int dmax = dg[0];
-#pragma unroll
-    for(int k = 1; k < R; ++k) dmax = dg[k] > dmax ? dg[k] : dmax;
-    int32_t cc[W][R];
-    VT av[W][R];
 #pragma unroll
     for(int p = 0; p < W; ++p)
     {
-        if(p < dmax)
+        bool any = false;
+#pragma unroll
+        for(int k = 0; k < R; ++k) any = any || in(p, k);
+        if(any)
         {
             ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
             ld_nt<R>(val + (int64_t)p * m + row, av[p]);
         }
 #pragma unroll
         for(int k = 0; k < R; ++k)
-            if(!(p < dg[k])) cc[p][k] = -1;
+            if(!in(p, k)) cc[p][k] = -1;
     }
-    XT d[R], s[R];
-#pragma unroll
-    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)dg[k] * m + row + k]);
-    ld_cvt<R>(r + row, s);
-    XT xv[W][R];
+}
+
+template <typename XT, int R>
+__device__ __forceinline__ void gather(const int32_t (&cc)[W][R], const XT* __restrict__ x, XT (&xv)[W][R])
+{
 #pragma unroll
     for(int p = 0; p < W; ++p)
 #pragma unroll
         for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
+}
+
+// s[k] -= sum over the marked slots of row k, ascending
+template <typename VT, typename XT, int R>
+__device__ __forceinline__ void sub_slots(const int32_t (&cc)[W][R], const VT (&av)[W][R], const XT (&xv)[W][R],
+                                          XT (&s)[R])
+{
 #pragma unroll
     for(int p = 0; p < W; ++p)
 #pragma unroll
         for(int k = 0; k < R; ++k)
             if(cc[p][k] >= 0) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+}
+
+template <typename VT, typename XT, int R>
+__device__ __forceinline__ void diag_vals(int64_t m, int64_t row, const VT* __restrict__ val, const int32_t (&ds)[R],
+                                          XT (&d)[R])
+{
+#pragma unroll
+    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)desc_diag(ds[k]) * m + row + k]);
+}
+
+// Zero-initial-guess forward sweep: only entries left of the diagonal
+// (earlier colours) are non-zero in x.
+template <typename VT, typename XT, typename RT, int R>
+__device__ __forceinline__ void row_fwd_zero(int64_t m, int64_t row, const int32_t* __restrict__ col,
+                                             const VT* __restrict__ val, const int32_t* __restrict__ desc,
+                                             const RT* __restrict__ r, XT* __restrict__ x)
+{
+    int32_t ds[R];
+    ld<R>(desc + row, ds);
+    int32_t cc[W][R];
+    VT av[W][R];
+    load_slots<VT, R>(m, row, col, val, cc, av, [&](int p, int k) { return p < desc_diag(ds[k]); });
+    XT d[R], s[R];
+    diag_vals<VT, XT, R>(m, row, val, ds, d);
+    ld_cvt<R>(r + row, s);
+    XT xv[W][R];
+    gather<XT, R>(cc, x, xv);
+    sub_slots<VT, XT, R>(cc, av, xv, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);
 }
 
-// Zero-initial-guess backward sweep: x = (x*d - U x) / d.
-template <typename VT, typename XT, int R>
+// Zero-initial-guess backward sweep: x = (x*d - U x) / d.  With STORE_U it
+// also keeps u = sum over (d, tl) of a*x -- the upper entries outside the last
+// colour, whose x the coarse-grid correction leaves alone -- for the
+// post-smoother's forward sweep (row_fwd_full_u).
+template <typename VT, typename XT, bool STORE_U, int R>
 __device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32_t* __restrict__ col,
-                                             const VT* __restrict__ val, const int32_t* __restrict__ dgi,
-                                             XT* __restrict__ x)
+                                             const VT* __restrict__ val, const int32_t* __restrict__ desc,
+                                             XT* __restrict__ x, XT* __restrict__ u)
 {
-    int32_t dg[R];
-    ld<R>(dgi + row, dg);
-    int dmin = dg[0];
-#pragma unroll
-    for(int k = 1; k < R; ++k) dmin = dg[k] < dmin ? dg[k] : dmin;
+    int32_t ds[R];
+    ld<R>(desc + row, ds);
     int32_t cc[W][R];
     VT av[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-    {
-        if(p > dmin)
-        {
-            ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
-            ld_nt<R>(val + (int64_t)p * m + row, av[p]);
-        }
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(!(p > dg[k])) cc[p][k] = -1;
-    }
+    load_slots<VT, R>(m, row, col, val, cc, av,
+                      [&](int p, int k) { return p > desc_diag(ds[k]) && p < desc_end(ds[k]); });
     XT d[R], s[R];
-#pragma unroll
-    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)dg[k] * m + row + k]);
+    diag_vals<VT, XT, R>(m, row, val, ds, d);
     ld<R>(x + row, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] * d[k];
     XT xv[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(cc[p][k] >= 0) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+    gather<XT, R>(cc, x, xv);
+    sub_slots<VT, XT, R>(cc, av, xv, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);
+    if(STORE_U)
+    {
+        XT uu[R];
+#pragma unroll
+        for(int k = 0; k < R; ++k) uu[k] = 0;
+#pragma unroll
+        for(int p = 0; p < W; ++p)
+#pragma unroll
+            for(int k = 0; k < R; ++k)
+                if(cc[p][k] >= 0 && p < desc_tail(ds[k])) uu[k] = fmaT(cvt<XT>(av[p][k]), xv[p][k], uu[k]);
+        st<R>(u + row, uu);
+    }
 }
 
 // Full forward sweep over the whole row in ascending column order; keeps
@@ -390,45 +424,71 @@ __device__ __forceinline__ void row_fwd_full(int64_t m, int64_t row, const int32
     if(STORE_T) st<R>(t + row, tt);
 }
 
+// The post-smoother's forward sweep for a row of a colour before the last,
+// when the coarse-grid correction changed only rows of the last colour:
+// x = (r - L x - u - T x) / d with u = t[row] from row_bwd_zero<STORE_U> (the
+// upper entries outside the last colour at their unchanged x) and T the
+// last-colour entries [tl, e) at their corrected x.  Reads the lower, the
+// diagonal and the last-colour slots only; t[row] becomes r - L x.
+template <typename VT, typename XT, typename RT, int R>
+__device__ __forceinline__ void row_fwd_full_u(int64_t m, int64_t row, const int32_t* __restrict__ col,
+                                               const VT* __restrict__ val, const int32_t* __restrict__ desc,
+                                               const RT* __restrict__ r, XT* __restrict__ x, XT* __restrict__ t)
+{
+    int32_t ds[R];
+    ld<R>(desc + row, ds);
+    int32_t cc[W][R];
+    VT av[W][R];
+    load_slots<VT, R>(m, row, col, val, cc, av, [&](int p, int k) {
+        return p < desc_diag(ds[k]) || (p >= desc_tail(ds[k]) && p < desc_end(ds[k]));
+    });
+    XT d[R], s[R], uu[R];
+    diag_vals<VT, XT, R>(m, row, val, ds, d);
+    ld_cvt<R>(r + row, s);
+    ld<R>(t + row, uu);
+    XT xv[W][R];
+    gather<XT, R>(cc, x, xv);
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            if(cc[p][k] >= 0 && p < desc_diag(ds[k])) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+    XT tt[R];
+#pragma unroll
+    for(int k = 0; k < R; ++k)
+    {
+        tt[k] = s[k];
+        s[k] = s[k] - uu[k];
+    }
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            if(cc[p][k] >= 0 && p > desc_diag(ds[k])) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+#pragma unroll
+    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
+    st<R>(x + row, s);
+    st<R>(t + row, tt);
+}
+
 // Full backward sweep from t: x = (t - U x) / d.
 template <typename VT, typename XT, int R>
 __device__ __forceinline__ void row_bwd_full(int64_t m, int64_t row, const int32_t* __restrict__ col,
-                                             const VT* __restrict__ val, const int32_t* __restrict__ dgi,
+                                             const VT* __restrict__ val, const int32_t* __restrict__ desc,
                                              const XT* __restrict__ t, XT* __restrict__ x)
 {
-    int32_t dg[R];
-    ld<R>(dgi + row, dg);
-    int dmin = dg[0];
-#pragma unroll
-    for(int k = 1; k < R; ++k) dmin = dg[k] < dmin ? dg[k] : dmin;
+    int32_t ds[R];
+    ld<R>(desc + row, ds);
     int32_t cc[W][R];
     VT av[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-    {
-        if(p > dmin)
-        {
-            ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
-            ld_nt<R>(val + (int64_t)p * m + row, av[p]);
-        }
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(!(p > dg[k])) cc[p][k] = -1;
-    }
+    load_slots<VT, R>(m, row, col, val, cc, av,
+                      [&](int p, int k) { return p > desc_diag(ds[k]) && p < desc_end(ds[k]); });
     XT d[R], s[R];
-#pragma unroll
-    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)dg[k] * m + row + k]);
+    diag_vals<VT, XT, R>(m, row, val, ds, d);
     ld<R>(t + row, s);
     XT xv[W][R];
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(cc[p][k] >= 0) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+    gather<XT, R>(cc, x, xv);
+    sub_slots<VT, XT, R>(cc, av, xv, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);

@@ -75,20 +75,20 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
 
 template <typename VT, typename XT, typename RT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
-                                                 const VT* __restrict__ val, const int32_t* __restrict__ dgi,
+                                                 const VT* __restrict__ val, const int32_t* __restrict__ desc,
                                                  const RT* __restrict__ r, XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    row_fwd_zero<VT, XT, RT, R>(m, row, col, val, dgi, r, x);
+    row_fwd_zero<VT, XT, RT, R>(m, row, col, val, desc, r, x);
 }
 
-template <typename VT, typename XT, int R, int BS>
+template <typename VT, typename XT, bool STORE_U, int R, int BS>
 __global__ __launch_bounds__(BS) void k_bwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
-                                                 const VT* __restrict__ val, const int32_t* __restrict__ dgi,
-                                                 XT* __restrict__ x, const int* gate)
+                                                 const VT* __restrict__ val, const int32_t* __restrict__ desc,
+                                                 XT* __restrict__ x, XT* __restrict__ u, const int* gate)
 {
     SWEEP_ROW();
-    row_bwd_zero<VT, XT, R>(m, row, col, val, dgi, x);
+    row_bwd_zero<VT, XT, STORE_U, R>(m, row, col, val, desc, x, u);
 }
 
 template <typename VT, typename XT, typename RT, bool STORE_T, int R, int BS>
@@ -100,13 +100,23 @@ __global__ __launch_bounds__(BS) void k_fwd_full(int64_t m, int64_t off, int64_t
     row_fwd_full<VT, XT, RT, STORE_T, R>(m, row, col, val, r, x, t);
 }
 
+template <typename VT, typename XT, typename RT, int R, int BS>
+__global__ __launch_bounds__(BS) void k_fwd_full_u(int64_t m, int64_t off, int64_t size,
+                                                   const int32_t* __restrict__ col, const VT* __restrict__ val,
+                                                   const int32_t* __restrict__ desc, const RT* __restrict__ r,
+                                                   XT* __restrict__ x, XT* __restrict__ t, const int* gate)
+{
+    SWEEP_ROW();
+    row_fwd_full_u<VT, XT, RT, R>(m, row, col, val, desc, r, x, t);
+}
+
 template <typename VT, typename XT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_bwd_full(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
-                                                 const VT* __restrict__ val, const int32_t* __restrict__ dgi,
+                                                 const VT* __restrict__ val, const int32_t* __restrict__ desc,
                                                  const XT* __restrict__ t, XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    row_bwd_full<VT, XT, R>(m, row, col, val, dgi, t, x);
+    row_bwd_full<VT, XT, R>(m, row, col, val, desc, t, x);
 }
 #undef SWEEP_ROW
 
@@ -452,9 +462,11 @@ static int per_color(const Level& L, int prec, int which, int c_first, int c_las
     });
 }
 
-// The four sweep kernels of a level: part 0 = zero-guess forward (colours
-// 0..n-1), 1 = zero-guess backward (n-2..0), 2 = forward keeping t (0..n-1),
-// 3 = backward from t (n-2..0).
+// The sweep kernels of a level: part 0 = zero-guess forward (colours 0..n-1),
+// 1 = zero-guess backward (n-2..0), 2 = forward keeping t (0..n-1), 3 =
+// backward from t (n-2..0), 4 = part 1 also keeping the upper sums outside
+// the last colour in t, 5 = part 2 reusing them (valid after part 4 when
+// only last-colour rows changed in between).
 template <typename VT, typename XT, typename RT>
 static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv, hipStream_t s, const int* gate)
 {
@@ -470,19 +482,28 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
         return per_color(L, pr, SH_FWD_ZERO, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             hipLaunchKernelGGL((k_fwd_zero<VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
-                               sz, L.ell_col, val, L.diag_idx, r, x, gate);
+                               sz, L.ell_col, val, L.desc, r, x, gate);
         });
     case 1:
+    case 4:
         if(last < 1) return 0;
         return per_color(L, pr, SH_BWD_ZERO, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
-            hipLaunchKernelGGL((k_bwd_zero<VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
-                               sz, L.ell_col, val, L.diag_idx, x, gate);
+            if(part == 4)
+                hipLaunchKernelGGL((k_bwd_zero<VT, XT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, x, t, gate);
+            else
+                hipLaunchKernelGGL((k_bwd_zero<VT, XT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, x, t, gate);
         });
     case 2:
+    case 5:
         return per_color(L, pr, SH_FWD_FULL, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
-            if(c < last)
+            if(c < last && part == 5)
+                hipLaunchKernelGGL((k_fwd_full_u<VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, r, x, t, gate);
+            else if(c < last)
                 hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
                                    (int64_t)L.offsets[c], sz, L.ell_col, val, r, x, t, gate);
             else
@@ -494,7 +515,7 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
         return per_color(L, pr, SH_BWD_FULL, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             hipLaunchKernelGGL((k_bwd_full<VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
-                               sz, L.ell_col, val, L.diag_idx, (const XT*)t, x, gate);
+                               sz, L.ell_col, val, L.desc, (const XT*)t, x, gate);
         });
     }
     set_error("invalid sweep part");
@@ -509,16 +530,17 @@ int launch_sweep(const Level& L, Prec prec, int part, const void* r, bool r_f64,
     return sweep_t<__half, float, float>(L, part, r, x, t, s, gate);
 }
 
-int launch_symgs_zero(const Level& L, Prec prec, const void* r, bool r_f64, void* x, hipStream_t s, const int* gate)
+int launch_symgs_zero(const Level& L, Prec prec, const void* r, bool r_f64, void* x, hipStream_t s, const int* gate,
+                      void* u)
 {
     HPCGX_TRY(launch_sweep(L, prec, 0, r, r_f64, x, nullptr, s, gate));
-    return launch_sweep(L, prec, 1, r, r_f64, x, nullptr, s, gate);
+    return launch_sweep(L, prec, u ? 4 : 1, r, r_f64, x, u, s, gate);
 }
 
 int launch_symgs(const Level& L, Prec prec, const void* r, bool r_f64, void* x, void* t, hipStream_t s,
-                 const int* gate)
+                 const int* gate, bool use_u)
 {
-    HPCGX_TRY(launch_sweep(L, prec, 2, r, r_f64, x, t, s, gate));
+    HPCGX_TRY(launch_sweep(L, prec, use_u ? 5 : 2, r, r_f64, x, t, s, gate));
     return launch_sweep(L, prec, 3, r, r_f64, x, t, s, gate);
 }
 

@@ -37,11 +37,17 @@ int launch_fold(int64_t nparts, const RedArgs& ra, hipStream_t s, const int* gat
 int launch_spmv(const Level& L, Prec prec, const void* x, void* y, hipStream_t s, const int* gate);
 int launch_spmv_f64x(const Level& L, Prec prec, const double* x, double* y, const RedArgs* ra, hipStream_t s,
                      const int* gate);
-// one of the four sweep kernels of a level (0 zero-guess fwd, 1 zero-guess bwd, 2 fwd keeping t, 3 bwd from t)
+// one sweep of a level: 0 zero-guess fwd, 1 zero-guess bwd, 2 fwd keeping t, 3 bwd from t,
+// 4 = 1 also keeping the upper sums outside the last colour in t, 5 = 2 reusing them
 int launch_sweep(const Level& L, Prec prec, int part, const void* r, bool r_f64, void* x, void* t, hipStream_t s,
                  const int* gate);
-int launch_symgs_zero(const Level& L, Prec prec, const void* r, bool r_f64, void* x, hipStream_t s, const int* gate);
-int launch_symgs(const Level& L, Prec prec, const void* r, bool r_f64, void* x, void* t, hipStream_t s, const int* gate);
+// u (may be null): also keep the upper sums outside the last colour, for a
+// post-smoother launch_symgs(..., use_u = true) after a coarse-grid
+// correction that changes only last-colour rows (Level::u_ok)
+int launch_symgs_zero(const Level& L, Prec prec, const void* r, bool r_f64, void* x, hipStream_t s, const int* gate,
+                      void* u = nullptr);
+int launch_symgs(const Level& L, Prec prec, const void* r, bool r_f64, void* x, void* t, hipStream_t s, const int* gate,
+                 bool use_u = false);
 int launch_restrict(const Level& F, Prec prec, const void* r, bool r_f64, const void* x, void* rc, hipStream_t s,
                     const int* gate);
 int launch_prolong(const Level& F, Prec prec, const void* xc, void* x, hipStream_t s, const int* gate);

@@ -178,7 +178,8 @@ template <typename VT>
 __global__ __launch_bounds__(256) void k_build_ell(Grid g, int64_t m, const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ iperm,
                                                    int32_t* __restrict__ ell_col, VT* __restrict__ ell_val,
-                                                   int32_t* __restrict__ diag_idx, double* __restrict__ inv_diag,
+                                                   int32_t* __restrict__ diag_idx, int32_t* __restrict__ desc,
+                                                   int32_t last_off, double* __restrict__ inv_diag,
                                                    float* __restrict__ diag_f)
 {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(256) void k_build_ell(Grid g, int64_t m, const int3
 #pragma unroll
         for(int k = (round & 1); k + 1 < W; k += 2) cswap(key[k], key[k + 1]);
     }
-    int dg = 0;
+    int dg = 0, tl = W, ne = 0;
 #pragma unroll
     for(int p = 0; p < W; ++p)
     {
@@ -211,8 +212,12 @@ __global__ __launch_bounds__(256) void k_build_ell(Grid g, int64_t m, const int3
         ell_col[(int64_t)p * m + r] = valid ? c : -1;
         ell_val[(int64_t)p * m + r] = (VT)(valid ? (isd ? 26.0f : -1.0f) : 0.0f);
         if(isd) dg = p;
+        if(valid) ne = p + 1;
+        if(valid && c >= last_off && p < tl) tl = p;
     }
+    if(tl > ne) tl = ne;
     diag_idx[r] = dg;
+    desc[r] = dg | (tl << 8) | (ne << 16);
     if(inv_diag) inv_diag[r] = 1.0 / 26.0;
     if(diag_f) diag_f[r] = 26.0f;
 }
@@ -460,12 +465,15 @@ int build_level(Level& L, int nx, int ny, int nz, bool with_f64, bool with_f16, 
     Grid g{nx, ny, nz};
     HPCGX_HIP(hipMalloc(&L.ell_col, sizeof(int32_t) * W * m));
     HPCGX_HIP(hipMalloc(&L.diag_idx, sizeof(int32_t) * m));
+    HPCGX_HIP(hipMalloc(&L.desc, sizeof(int32_t) * m));
+    const int32_t last_off = L.offsets[L.ncolors - 1];
     if(with_f64)
     {
         HPCGX_HIP(hipMalloc(&L.ell_val, sizeof(double) * W * m));
         HPCGX_HIP(hipMalloc(&L.inv_diag, sizeof(double) * m));
         hipLaunchKernelGGL(k_build_ell<double>, dim3(grid_of(m, 256)), dim3(256), 0, s, g, m, L.perm,
-                           L.iperm_tmp, L.ell_col, L.ell_val, L.diag_idx, L.inv_diag, (float*)nullptr);
+                           L.iperm_tmp, L.ell_col, L.ell_val, L.diag_idx, L.desc, last_off, L.inv_diag,
+                           (float*)nullptr);
         HPCGX_HIP(hipGetLastError());
     }
     if(with_f16)
@@ -473,7 +481,8 @@ int build_level(Level& L, int nx, int ny, int nz, bool with_f64, bool with_f16, 
         HPCGX_HIP(hipMalloc(&L.ell_h, sizeof(__half) * W * m));
         HPCGX_HIP(hipMalloc(&L.diag_f, sizeof(float) * m));
         hipLaunchKernelGGL(k_build_ell<__half>, dim3(grid_of(m, 256)), dim3(256), 0, s, g, m, L.perm,
-                           L.iperm_tmp, L.ell_col, L.ell_h, L.diag_idx, (double*)nullptr, L.diag_f);
+                           L.iperm_tmp, L.ell_col, L.ell_h, L.diag_idx, L.desc, last_off, (double*)nullptr,
+                           L.diag_f);
         HPCGX_HIP(hipGetLastError());
     }
     HPCGX_HIP(hipStreamSynchronize(s));
@@ -492,6 +501,14 @@ int build_transfer(Level& F, Level& C, hipStream_t s)
     hipLaunchKernelGGL(k_transfer, dim3(grid_of(F.mc, 256)), dim3(256), 0, s, gc, F.nx, F.ny, F.mc, F.perm,
                        C.perm, F.f2c_nat, F.f2c_fine, F.f2c_coarse);
     HPCGX_HIP(hipGetLastError());
+    // are all injected rows in the last colour? (true for the parity-class
+    // colourings JPL produces on these grids; checked, not assumed)
+    std::vector<int32_t> h((size_t)F.mc);
+    HPCGX_HIP(hipMemcpyAsync(h.data(), F.f2c_fine, sizeof(int32_t) * F.mc, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+    const int32_t last_off = F.offsets[F.ncolors - 1];
+    F.u_ok = F.ncolors > 1;
+    for(int32_t v : h) F.u_ok = F.u_ok && v >= last_off;
     return 0;
 }
 
@@ -513,6 +530,7 @@ void free_level(Level& L)
     f(L.ell_val);
     f(L.ell_h);
     f(L.diag_idx);
+    f(L.desc);
     f(L.inv_diag);
     f(L.diag_f);
     f(L.perm);

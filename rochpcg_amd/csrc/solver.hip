@@ -195,11 +195,13 @@ int mg(Problem& P, int l, Prec prec, const void* r, bool r_f64, void* x, const i
     if(l + 1 < P.nlevels)
     {
         LevelVecs& C = V[l + 1];
-        HPCGX_TRY(launch_symgs_zero(F, prec, r, r_f64, x, P.s, gate));
+        // V[l].t carries the pre-smoother's upper sums to the post-smoother
+        // when the correction touches only last-colour rows (F.u_ok)
+        HPCGX_TRY(launch_symgs_zero(F, prec, r, r_f64, x, P.s, gate, F.u_ok ? V[l].t : nullptr));
         HPCGX_TRY(launch_restrict(F, prec, r, r_f64, x, C.r, P.s, gate));
         HPCGX_TRY(mg(P, l + 1, prec, C.r, prec == PREC_F64, C.x, gate));
         HPCGX_TRY(launch_prolong(F, prec, C.x, x, P.s, gate));
-        HPCGX_TRY(launch_symgs(F, prec, r, r_f64, x, V[l].t, P.s, gate));
+        HPCGX_TRY(launch_symgs(F, prec, r, r_f64, x, V[l].t, P.s, gate, F.u_ok));
     }
     else
     {

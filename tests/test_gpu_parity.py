@@ -46,6 +46,17 @@ def test_setup_bit_exact(hip, oracle_mod, dims):
         assert np.array_equal(g["ell_val"].reshape(27, m).T, o["vals"]), f"ell_val level {lv}"
         assert np.array_equal(g["ell_f16"].reshape(27, m).T.astype(np.float64), o["vals"])
         assert np.array_equal(g["diag_idx"], o["diag"])
+        # row descriptor: diagonal slot, first slot holding a column of the
+        # last colour (end of row if none), number of valid slots
+        oc = o["cols"]
+        valid = oc >= 0
+        ne = valid.sum(axis=1)
+        in_last = valid & (oc >= offsets[L.ncolors - 1])
+        tl = np.where(in_last.any(axis=1), np.argmax(in_last, axis=1), ne)
+        desc = g["desc"]
+        assert np.array_equal(desc & 255, o["diag"]), f"desc diag level {lv}"
+        assert np.array_equal((desc >> 8) & 255, tl), f"desc tail level {lv}"
+        assert np.array_equal(desc >> 16, ne), f"desc end level {lv}"
         assert np.all(g["inv_diag"] == 1.0 / 26.0)
         if lv + 1 < 4:
             # oracle f2c is indexed by the coarse multicolour row
@@ -149,6 +160,35 @@ def test_cg_history_fp64(hip, oracle_mod, dims, use_graph):
     h, hr = res["history"], ref["history"]
     assert abs(h[0] - hr[0]) <= 1e-14 * hr[0]
     assert_history_close(h, hr)
+    G.close()
+
+
+def test_vcycle_with_and_without_kept_upper_sums(hip, oracle_mod):
+    """16x8x8 over 4 levels: levels 0 and 1 inject only into their last
+    colour, so the post-smoother reuses the pre-smoother's upper sums there;
+    level 2 (4x2x2, 10 colours) injects into other colours as well and runs
+    the plain forward sweep.  Both paths must give the oracle's V-cycle and
+    CG history, in FP64 and in the mixed store."""
+    dims = (16, 8, 8)
+    G = _hip_problem(hip, dims)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    kept = []
+    for lv in range(3):
+        g = G.level_arrays(lv)
+        _, offsets, _ = G.color_table(lv)
+        kept.append(bool(np.all(g["f2c_fine"] >= offsets[G.levels[lv].ncolors - 1])))
+    assert kept == [True, True, False]
+    rng = np.random.default_rng(11)
+    r = rng.uniform(-1, 1, G.m)
+    dr, dz = hip.DeviceArray.from_host(r), hip.DeviceArray(G.m)
+    G.mg(dr, dz)
+    assert _rel(dz.host(), O.mg(r)) <= 1e-12
+    res = G.cg(max_iter=40, tol=0.0, use_graph=True)
+    ref = O.cg(O.rhs_ones(), max_iter=40)
+    assert_history_close(res["history"], ref["history"])
+    G.vec["x"].upload(np.zeros(G.m))
+    mx = G.cg(max_iter=200, tol=1e-9, mode=hip.CG_MIXED, use_graph=True)
+    assert mx["normr"] / mx["normr0"] <= 1e-9
     G.close()
 
 
