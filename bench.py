@@ -24,6 +24,23 @@ METRIC = "HPCG GFLOP/s + HBM TB/s: mixed vs FP64 CG to rel. residual 1e-9, 1 GPU
 HBM_PEAK_GBPS = 8000.0
 
 
+def profiled_traffic(dims):
+    """HBM bytes per launch of the fused SpMV + p'Ap kernel (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated) from the committed rocprofv3 summary of this
+    grid, or None when there is none."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{dims[0]}x{dims[1]}x{dims[2]}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                m = re.match(r"^roofline traffic \(k_spmv, HBM bytes per launch\): (\d+)", line)
+                if m:
+                    return int(m.group(1))
+    except OSError:
+        pass
+    return None
+
+
 def level_rows(dims, levels):
     out = []
     d = list(dims)
@@ -253,7 +270,7 @@ def main():
         "roofline": {"kernel": "k_spmv (outer SpMV + fused p'Ap), level 0",
                      "bound": "hbm", "achieved": round(spmv_gbps, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(spmv_gbps / HBM_PEAK_GBPS, 4),
-                     "traffic": None, "bytes_per_launch": spmv_bytes(G.m),
+                     "traffic": profiled_traffic(dims), "bytes_per_launch": spmv_bytes(G.m),
                      "avg_launch_ms": round(spmv_ms, 4)},
     }
 

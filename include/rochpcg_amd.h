@@ -180,7 +180,7 @@ int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps,
  * blocks, ComputeSPMV.cpp:262, ComputeSYMGS.cpp:395): op 0 = SpMV,
  * 1 = zero-guess forward sweep, 2 = zero-guess backward sweep, 3 = forward
  * sweep, 4 = backward sweep, 5 = restriction; block in {64,128,256} threads,
- * rows in {1,2,4} consecutive rows per thread (restriction: 1).  level -1 /
+ * rows in {1,2} consecutive rows per thread (restriction: 1).  level -1 /
  * op -1 apply to all; prec 0 = f64 kernels, 1 = mixed kernels.  Colour
  * blocks not divisible by `rows` run with 1. */
 int hpcgx_set_shape(hpcgx_problem* p, int prec, int level, int op, int block, int rows);

@@ -2,7 +2,7 @@
 
 For every level of a problem and every tunable operator (SpMV, the four
 SymGS sweep kernels, the fused restriction) and both precisions, time each
-compiled launch shape -- rows per thread {1, 2, 4} x threads per block
+compiled launch shape -- rows per thread {1, 2} x threads per block
 {64, 128, 256} -- with HIP events on the solver stream and keep the fastest.
 Results are cached as JSON keyed by the level's grid and the precision, so a
 later run on the same grids applies them without timing.  The reference has
@@ -18,7 +18,7 @@ from . import (OP_BWD_FULL, OP_BWD_ZERO, OP_FWD_FULL, OP_FWD_ZERO, OP_RESTRICT, 
                SH_BWD_FULL, SH_BWD_ZERO, SH_FWD_FULL, SH_FWD_ZERO, SH_RESTRICT, SH_SPMV,
                STORE_F64, STORE_MIXED)
 
-SHAPES = [(block, rows) for rows in (1, 2, 4) for block in (64, 128, 256)]
+SHAPES = [(block, rows) for rows in (1, 2) for block in (64, 128, 256)]
 OPS = {  # tunable operator -> timing operator
     SH_SPMV: OP_SPMV, SH_FWD_ZERO: OP_FWD_ZERO, SH_BWD_ZERO: OP_BWD_ZERO,
     SH_FWD_FULL: OP_FWD_FULL, SH_BWD_FULL: OP_BWD_FULL, SH_RESTRICT: OP_RESTRICT,

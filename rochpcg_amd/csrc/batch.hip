@@ -74,7 +74,7 @@ __global__ __launch_bounds__(BS) void kb_fwd_zero(const BSeg* __restrict__ segs,
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    row_fwd_zero<VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x);
+    row_fwd_zero<W, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x);
 }
 
 template <typename VT, typename XT, int BS>
@@ -84,9 +84,9 @@ __global__ __launch_bounds__(BS) void kb_bwd_zero(const BSeg* __restrict__ segs,
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
     if(S.t)
-        row_bwd_zero<VT, XT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, (XT*)S.t);
+        row_bwd_zero<W, VT, XT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, (XT*)S.t);
     else
-        row_bwd_zero<VT, XT, false, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, nullptr);
+        row_bwd_zero<W, VT, XT, false, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, nullptr);
 }
 
 template <typename VT, typename XT, typename RT, int BS>
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(BS) void kb_fwd_full(const BSeg* __restrict__ segs,
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
     if(S.use_u)
-        row_fwd_full_u<VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x,
+        row_fwd_full_u<W, W, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x,
                                       (XT*)S.t);
     else if(S.t)
         row_fwd_full<VT, XT, RT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(BS) void kb_bwd_full(const BSeg* __restrict__ segs,
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    row_bwd_full<VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const XT*)S.t, (XT*)S.x);
+    row_bwd_full<W, VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const XT*)S.t, (XT*)S.x);
 }
 
 template <typename VT, typename XT, typename RT, int BS>

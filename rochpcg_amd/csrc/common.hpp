@@ -54,7 +54,17 @@ enum Prec : int
 struct Shape
 {
     int block = 256;   // threads per workgroup: 64, 128 or 256
-    int rows = 1;      // consecutive rows per thread: 1, 2 or 4
+    int rows = 1;      // consecutive rows per thread: 1 or 2
+};
+
+// Slot windows of the sweeps (see load_window): per colour and rows-per-thread
+// R in {1, 2}, the largest window a thread needs
+enum WinKind : int
+{
+    WIN_LOWER = 0,   // [0, d): max over threads of max d
+    WIN_UPPER = 1,   // (d, e): max over threads of max e - min (d+1)
+    WIN_TAIL = 2,    // [tl, e): max over threads of max e - min tl
+    WIN_COUNT = 3
 };
 
 // tunable operators of a level
@@ -102,6 +112,7 @@ struct Level
     // only last-colour rows, so the post-smoother may reuse the pre-smoother's
     // upper sums over the other colours
     bool u_ok = false;
+    unsigned char win[WIN_COUNT][2][MAX_COLORS] = {};   // [kind][R-1][colour]
 
     Shape shp[2][SH_COUNT];   // [precision][operator]
 };

@@ -257,48 +257,55 @@ __device__ __forceinline__ int desc_diag(int32_t v) { return v & 255; }
 __device__ __forceinline__ int desc_tail(int32_t v) { return (v >> 8) & 255; }
 __device__ __forceinline__ int desc_end(int32_t v) { return (v >> 16) & 255; }
 
-// Loads slot p of the thread's R rows where in(p, k) holds for some k, and
-// marks the slots of row k outside in(p, k) with column -1.
-template <typename VT, int R, typename In>
-__device__ __forceinline__ void load_slots(int64_t m, int64_t row, const int32_t* __restrict__ col,
-                                           const VT* __restrict__ val, int32_t (&cc)[W][R], VT (&av)[W][R], In in)
+// Slot windows.  A sweep touches one or two contiguous slot ranges of each
+// row ([0, d), (d, e) or [tl, e)).  A thread's window starts at slot `start`
+// (the smallest start over its R rows) and spans NS slots: NS is a
+// compile-time bound the host picks per colour from the rows' descriptors
+// (Level::win), so light colours keep few registers live and run at higher
+// occupancy.  Slot start+q of row k is used iff lo[k] <= start+q < hi[k];
+// unused slots carry column -1.
+template <int NS, typename VT, int R>
+__device__ __forceinline__ void load_window(int64_t m, int64_t row, const int32_t* __restrict__ col,
+                                            const VT* __restrict__ val, int start, const int (&lo)[R],
+                                            const int (&hi)[R], int32_t (&cc)[NS][R], VT (&av)[NS][R])
 {
 #pragma unroll
-    for(int p = 0; p < W; ++p)
+    for(int q = 0; q < NS; ++q)
     {
+        const int p = start + q;
         bool any = false;
 #pragma unroll
-        for(int k = 0; k < R; ++k) any = any || in(p, k);
+        for(int k = 0; k < R; ++k) any = any || (p >= lo[k] && p < hi[k]);
         if(any)
         {
-            ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
-            ld_nt<R>(val + (int64_t)p * m + row, av[p]);
+            ld_nt<R>(col + (int64_t)p * m + row, cc[q]);
+            ld_nt<R>(val + (int64_t)p * m + row, av[q]);
         }
 #pragma unroll
         for(int k = 0; k < R; ++k)
-            if(!in(p, k)) cc[p][k] = -1;
+            if(!(p >= lo[k] && p < hi[k])) cc[q][k] = -1;
     }
 }
 
-template <typename XT, int R>
-__device__ __forceinline__ void gather(const int32_t (&cc)[W][R], const XT* __restrict__ x, XT (&xv)[W][R])
+template <int NS, typename XT, int R>
+__device__ __forceinline__ void gather(const int32_t (&cc)[NS][R], const XT* __restrict__ x, XT (&xv)[NS][R])
 {
 #pragma unroll
-    for(int p = 0; p < W; ++p)
+    for(int q = 0; q < NS; ++q)
 #pragma unroll
-        for(int k = 0; k < R; ++k) xv[p][k] = (cc[p][k] >= 0) ? x[cc[p][k]] : (XT)0;
+        for(int k = 0; k < R; ++k) xv[q][k] = (cc[q][k] >= 0) ? x[cc[q][k]] : (XT)0;
 }
 
-// s[k] -= sum over the marked slots of row k, ascending
-template <typename VT, typename XT, int R>
-__device__ __forceinline__ void sub_slots(const int32_t (&cc)[W][R], const VT (&av)[W][R], const XT (&xv)[W][R],
-                                          XT (&s)[R])
+// s[k] -= sum over the used slots of row k, ascending
+template <int NS, typename VT, typename XT, int R>
+__device__ __forceinline__ void sub_window(const int32_t (&cc)[NS][R], const VT (&av)[NS][R],
+                                           const XT (&xv)[NS][R], XT (&s)[R])
 {
 #pragma unroll
-    for(int p = 0; p < W; ++p)
+    for(int q = 0; q < NS; ++q)
 #pragma unroll
         for(int k = 0; k < R; ++k)
-            if(cc[p][k] >= 0) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+            if(cc[q][k] >= 0) s[k] = fmaT(-cvt<XT>(av[q][k]), xv[q][k], s[k]);
 }
 
 template <typename VT, typename XT, int R>
@@ -309,52 +316,76 @@ __device__ __forceinline__ void diag_vals(int64_t m, int64_t row, const VT* __re
     for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)desc_diag(ds[k]) * m + row + k]);
 }
 
+template <int R>
+__device__ __forceinline__ int min_of(const int (&v)[R])
+{
+    int r = v[0];
+#pragma unroll
+    for(int k = 1; k < R; ++k) r = v[k] < r ? v[k] : r;
+    return r;
+}
+
 // Zero-initial-guess forward sweep: only entries left of the diagonal
-// (earlier colours) are non-zero in x.
-template <typename VT, typename XT, typename RT, int R>
+// (earlier colours) are non-zero in x.  Window [0, d), NS >= max d.
+template <int NS, typename VT, typename XT, typename RT, int R>
 __device__ __forceinline__ void row_fwd_zero(int64_t m, int64_t row, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, const int32_t* __restrict__ desc,
                                              const RT* __restrict__ r, XT* __restrict__ x)
 {
     int32_t ds[R];
     ld<R>(desc + row, ds);
-    int32_t cc[W][R];
-    VT av[W][R];
-    load_slots<VT, R>(m, row, col, val, cc, av, [&](int p, int k) { return p < desc_diag(ds[k]); });
+    int lo[R], hi[R];
+#pragma unroll
+    for(int k = 0; k < R; ++k)
+    {
+        lo[k] = 0;
+        hi[k] = desc_diag(ds[k]);
+    }
+    int32_t cc[NS][R];
+    VT av[NS][R];
+    load_window<NS, VT, R>(m, row, col, val, 0, lo, hi, cc, av);
     XT d[R], s[R];
     diag_vals<VT, XT, R>(m, row, val, ds, d);
     ld_cvt<R>(r + row, s);
-    XT xv[W][R];
-    gather<XT, R>(cc, x, xv);
-    sub_slots<VT, XT, R>(cc, av, xv, s);
+    XT xv[NS][R];
+    gather<NS, XT, R>(cc, x, xv);
+    sub_window<NS, VT, XT, R>(cc, av, xv, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);
 }
 
-// Zero-initial-guess backward sweep: x = (x*d - U x) / d.  With STORE_U it
-// also keeps u = sum over (d, tl) of a*x -- the upper entries outside the last
-// colour, whose x the coarse-grid correction leaves alone -- for the
-// post-smoother's forward sweep (row_fwd_full_u).
-template <typename VT, typename XT, bool STORE_U, int R>
+// Zero-initial-guess backward sweep: x = (x*d - U x) / d over the window
+// (d, e), NS >= max e - min(d+1).  With STORE_U it also keeps u = sum over
+// (d, tl) of a*x -- the upper entries outside the last colour, whose x the
+// coarse-grid correction leaves alone -- for the post-smoother's forward
+// sweep (row_fwd_full_u).
+template <int NS, typename VT, typename XT, bool STORE_U, int R>
 __device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, const int32_t* __restrict__ desc,
                                              XT* __restrict__ x, XT* __restrict__ u)
 {
     int32_t ds[R];
     ld<R>(desc + row, ds);
-    int32_t cc[W][R];
-    VT av[W][R];
-    load_slots<VT, R>(m, row, col, val, cc, av,
-                      [&](int p, int k) { return p > desc_diag(ds[k]) && p < desc_end(ds[k]); });
+    int lo[R], hi[R];
+#pragma unroll
+    for(int k = 0; k < R; ++k)
+    {
+        lo[k] = desc_diag(ds[k]) + 1;
+        hi[k] = desc_end(ds[k]);
+    }
+    const int start = min_of<R>(lo);
+    int32_t cc[NS][R];
+    VT av[NS][R];
+    load_window<NS, VT, R>(m, row, col, val, start, lo, hi, cc, av);
     XT d[R], s[R];
     diag_vals<VT, XT, R>(m, row, val, ds, d);
     ld<R>(x + row, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] * d[k];
-    XT xv[W][R];
-    gather<XT, R>(cc, x, xv);
-    sub_slots<VT, XT, R>(cc, av, xv, s);
+    XT xv[NS][R];
+    gather<NS, XT, R>(cc, x, xv);
+    sub_window<NS, VT, XT, R>(cc, av, xv, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);
@@ -364,10 +395,11 @@ __device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32
 #pragma unroll
         for(int k = 0; k < R; ++k) uu[k] = 0;
 #pragma unroll
-        for(int p = 0; p < W; ++p)
+        for(int q = 0; q < NS; ++q)
 #pragma unroll
             for(int k = 0; k < R; ++k)
-                if(cc[p][k] >= 0 && p < desc_tail(ds[k])) uu[k] = fmaT(cvt<XT>(av[p][k]), xv[p][k], uu[k]);
+                if(cc[q][k] >= 0 && start + q < desc_tail(ds[k]))
+                    uu[k] = fmaT(cvt<XT>(av[q][k]), xv[q][k], uu[k]);
         st<R>(u + row, uu);
     }
 }
@@ -429,30 +461,37 @@ __device__ __forceinline__ void row_fwd_full(int64_t m, int64_t row, const int32
 // x = (r - L x - u - T x) / d with u = t[row] from row_bwd_zero<STORE_U> (the
 // upper entries outside the last colour at their unchanged x) and T the
 // last-colour entries [tl, e) at their corrected x.  Reads the lower, the
-// diagonal and the last-colour slots only; t[row] becomes r - L x.
-template <typename VT, typename XT, typename RT, int R>
+// diagonal and the last-colour slots only (windows of NA >= max d and
+// NB >= max e - min tl slots); t[row] becomes r - L x.
+template <int NA, int NB, typename VT, typename XT, typename RT, int R>
 __device__ __forceinline__ void row_fwd_full_u(int64_t m, int64_t row, const int32_t* __restrict__ col,
                                                const VT* __restrict__ val, const int32_t* __restrict__ desc,
                                                const RT* __restrict__ r, XT* __restrict__ x, XT* __restrict__ t)
 {
     int32_t ds[R];
     ld<R>(desc + row, ds);
-    int32_t cc[W][R];
-    VT av[W][R];
-    load_slots<VT, R>(m, row, col, val, cc, av, [&](int p, int k) {
-        return p < desc_diag(ds[k]) || (p >= desc_tail(ds[k]) && p < desc_end(ds[k]));
-    });
+    int alo[R], ahi[R], blo[R], bhi[R];
+#pragma unroll
+    for(int k = 0; k < R; ++k)
+    {
+        alo[k] = 0;
+        ahi[k] = desc_diag(ds[k]);
+        blo[k] = desc_tail(ds[k]);
+        bhi[k] = desc_end(ds[k]);
+    }
+    const int bstart = min_of<R>(blo);
+    int32_t ca[NA][R], cb[NB][R];
+    VT aa[NA][R], ab[NB][R];
+    load_window<NA, VT, R>(m, row, col, val, 0, alo, ahi, ca, aa);
+    load_window<NB, VT, R>(m, row, col, val, bstart, blo, bhi, cb, ab);
     XT d[R], s[R], uu[R];
     diag_vals<VT, XT, R>(m, row, val, ds, d);
     ld_cvt<R>(r + row, s);
     ld<R>(t + row, uu);
-    XT xv[W][R];
-    gather<XT, R>(cc, x, xv);
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(cc[p][k] >= 0 && p < desc_diag(ds[k])) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+    XT xa[NA][R], xb[NB][R];
+    gather<NA, XT, R>(ca, x, xa);
+    gather<NB, XT, R>(cb, x, xb);
+    sub_window<NA, VT, XT, R>(ca, aa, xa, s);
     XT tt[R];
 #pragma unroll
     for(int k = 0; k < R; ++k)
@@ -460,35 +499,38 @@ __device__ __forceinline__ void row_fwd_full_u(int64_t m, int64_t row, const int
         tt[k] = s[k];
         s[k] = s[k] - uu[k];
     }
-#pragma unroll
-    for(int p = 0; p < W; ++p)
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(cc[p][k] >= 0 && p > desc_diag(ds[k])) s[k] = fmaT(-cvt<XT>(av[p][k]), xv[p][k], s[k]);
+    sub_window<NB, VT, XT, R>(cb, ab, xb, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);
     st<R>(t + row, tt);
 }
 
-// Full backward sweep from t: x = (t - U x) / d.
-template <typename VT, typename XT, int R>
+// Full backward sweep from t: x = (t - U x) / d over the window (d, e).
+template <int NS, typename VT, typename XT, int R>
 __device__ __forceinline__ void row_bwd_full(int64_t m, int64_t row, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, const int32_t* __restrict__ desc,
                                              const XT* __restrict__ t, XT* __restrict__ x)
 {
     int32_t ds[R];
     ld<R>(desc + row, ds);
-    int32_t cc[W][R];
-    VT av[W][R];
-    load_slots<VT, R>(m, row, col, val, cc, av,
-                      [&](int p, int k) { return p > desc_diag(ds[k]) && p < desc_end(ds[k]); });
+    int lo[R], hi[R];
+#pragma unroll
+    for(int k = 0; k < R; ++k)
+    {
+        lo[k] = desc_diag(ds[k]) + 1;
+        hi[k] = desc_end(ds[k]);
+    }
+    const int start = min_of<R>(lo);
+    int32_t cc[NS][R];
+    VT av[NS][R];
+    load_window<NS, VT, R>(m, row, col, val, start, lo, hi, cc, av);
     XT d[R], s[R];
     diag_vals<VT, XT, R>(m, row, val, ds, d);
     ld<R>(t + row, s);
-    XT xv[W][R];
-    gather<XT, R>(cc, x, xv);
-    sub_slots<VT, XT, R>(cc, av, xv, s);
+    XT xv[NS][R];
+    gather<NS, XT, R>(cc, x, xv);
+    sub_window<NS, VT, XT, R>(cc, av, xv, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);

@@ -73,22 +73,22 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
     if(i >= size) return;                                                             \
     const int64_t row = off + i;
 
-template <typename VT, typename XT, typename RT, int R, int BS>
+template <int NS, typename VT, typename XT, typename RT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ desc,
                                                  const RT* __restrict__ r, XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    row_fwd_zero<VT, XT, RT, R>(m, row, col, val, desc, r, x);
+    row_fwd_zero<NS, VT, XT, RT, R>(m, row, col, val, desc, r, x);
 }
 
-template <typename VT, typename XT, bool STORE_U, int R, int BS>
+template <int NS, typename VT, typename XT, bool STORE_U, int R, int BS>
 __global__ __launch_bounds__(BS) void k_bwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ desc,
                                                  XT* __restrict__ x, XT* __restrict__ u, const int* gate)
 {
     SWEEP_ROW();
-    row_bwd_zero<VT, XT, STORE_U, R>(m, row, col, val, desc, x, u);
+    row_bwd_zero<NS, VT, XT, STORE_U, R>(m, row, col, val, desc, x, u);
 }
 
 template <typename VT, typename XT, typename RT, bool STORE_T, int R, int BS>
@@ -100,23 +100,23 @@ __global__ __launch_bounds__(BS) void k_fwd_full(int64_t m, int64_t off, int64_t
     row_fwd_full<VT, XT, RT, STORE_T, R>(m, row, col, val, r, x, t);
 }
 
-template <typename VT, typename XT, typename RT, int R, int BS>
+template <int NA, int NB, typename VT, typename XT, typename RT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_fwd_full_u(int64_t m, int64_t off, int64_t size,
                                                    const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                    const int32_t* __restrict__ desc, const RT* __restrict__ r,
                                                    XT* __restrict__ x, XT* __restrict__ t, const int* gate)
 {
     SWEEP_ROW();
-    row_fwd_full_u<VT, XT, RT, R>(m, row, col, val, desc, r, x, t);
+    row_fwd_full_u<NA, NB, VT, XT, RT, R>(m, row, col, val, desc, r, x, t);
 }
 
-template <typename VT, typename XT, int R, int BS>
+template <int NS, typename VT, typename XT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_bwd_full(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ desc,
                                                  const XT* __restrict__ t, XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    row_bwd_full<VT, XT, R>(m, row, col, val, desc, t, x);
+    row_bwd_full<NS, VT, XT, R>(m, row, col, val, desc, t, x);
 }
 #undef SWEEP_ROW
 
@@ -351,9 +351,6 @@ static int with_shape(const Shape& sh, F&& f)
     case 2064: return f(integral_constant<int, 2>{}, integral_constant<int, 64>{});
     case 2128: return f(integral_constant<int, 2>{}, integral_constant<int, 128>{});
     case 2256: return f(integral_constant<int, 2>{}, integral_constant<int, 256>{});
-    case 4064: return f(integral_constant<int, 4>{}, integral_constant<int, 64>{});
-    case 4128: return f(integral_constant<int, 4>{}, integral_constant<int, 128>{});
-    case 4256: return f(integral_constant<int, 4>{}, integral_constant<int, 256>{});
     default: break;
     }
     set_error("unsupported launch shape (rows " + std::to_string(sh.rows) + ", block " + std::to_string(sh.block) + ")");
@@ -362,7 +359,7 @@ static int with_shape(const Shape& sh, F&& f)
 
 bool shape_supported(int rows, int block)
 {
-    return (rows == 1 || rows == 2 || rows == 4) && (block == 64 || block == 128 || block == 256);
+    return (rows == 1 || rows == 2) && (block == 64 || block == 128 || block == 256);
 }
 
 // R rows per thread need every colour block (and m) to be R-aligned.
@@ -440,6 +437,29 @@ int launch_spmv_f64x(const Level& L, Prec prec, const double* x, double* y, cons
     return spmv_t<__half, double, double>(L, x, y, ra, s, gate);
 }
 
+// compile-time slot-window bounds: 4, 16 or all 27 slots (tails: 8 or 27)
+template <typename F>
+static void with_window(int w, F&& f)
+{
+    using std::integral_constant;
+    if(w <= 4)
+        f(integral_constant<int, 4>{});
+    else if(w <= 16)
+        f(integral_constant<int, 16>{});
+    else
+        f(integral_constant<int, W>{});
+}
+
+template <typename F>
+static void with_tail_window(int w, F&& f)
+{
+    using std::integral_constant;
+    if(w <= 8)
+        f(integral_constant<int, 8>{});
+    else
+        f(integral_constant<int, W>{});
+}
+
 // one launch per colour c in [c0, c1) (step +1) or (c0, c1] (step -1)
 template <typename Launch>
 static int per_color(const Level& L, int prec, int which, int c_first, int c_last, int step, Launch&& launch)
@@ -476,33 +496,47 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
     const VT* val = vals_of<VT>(L);
     const int last = L.ncolors - 1;
     const int pr = std::is_same<VT, double>::value ? PREC_F64 : PREC_MIXED;
+    using std::integral_constant;
     switch(part)
     {
     case 0:
         return per_color(L, pr, SH_FWD_ZERO, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
-            hipLaunchKernelGGL((k_fwd_zero<VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
-                               sz, L.ell_col, val, L.desc, r, x, gate);
+            with_window(L.win[WIN_LOWER][R - 1][c], [&](auto NN) {
+                constexpr int NS = decltype(NN)::value;
+                hipLaunchKernelGGL((k_fwd_zero<NS, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, r, x, gate);
+            });
         });
     case 1:
     case 4:
         if(last < 1) return 0;
         return per_color(L, pr, SH_BWD_ZERO, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
-            if(part == 4)
-                hipLaunchKernelGGL((k_bwd_zero<VT, XT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, x, t, gate);
-            else
-                hipLaunchKernelGGL((k_bwd_zero<VT, XT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, x, t, gate);
+            with_window(L.win[WIN_UPPER][R - 1][c], [&](auto NN) {
+                constexpr int NS = decltype(NN)::value;
+                if(part == 4)
+                    hipLaunchKernelGGL((k_bwd_zero<NS, VT, XT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                       (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, x, t, gate);
+                else
+                    hipLaunchKernelGGL((k_bwd_zero<NS, VT, XT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                       (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, x, t, gate);
+            });
         });
     case 2:
     case 5:
         return per_color(L, pr, SH_FWD_FULL, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             if(c < last && part == 5)
-                hipLaunchKernelGGL((k_fwd_full_u<VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, r, x, t, gate);
+            {
+                with_window(L.win[WIN_LOWER][R - 1][c], [&](auto NA_) {
+                    with_tail_window(L.win[WIN_TAIL][R - 1][c], [&](auto NB_) {
+                        constexpr int NA = decltype(NA_)::value, NB = decltype(NB_)::value;
+                        hipLaunchKernelGGL((k_fwd_full_u<NA, NB, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                           (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, r, x, t, gate);
+                    });
+                });
+            }
             else if(c < last)
                 hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
                                    (int64_t)L.offsets[c], sz, L.ell_col, val, r, x, t, gate);
@@ -514,8 +548,11 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
         if(last < 1) return 0;
         return per_color(L, pr, SH_BWD_FULL, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
-            hipLaunchKernelGGL((k_bwd_full<VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m, (int64_t)L.offsets[c],
-                               sz, L.ell_col, val, L.desc, (const XT*)t, x, gate);
+            with_window(L.win[WIN_UPPER][R - 1][c], [&](auto NN) {
+                constexpr int NS = decltype(NN)::value;
+                hipLaunchKernelGGL((k_bwd_full<NS, VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, (const XT*)t, x, gate);
+            });
         });
     }
     set_error("invalid sweep part");

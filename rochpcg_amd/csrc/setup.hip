@@ -16,6 +16,7 @@
 #include "common.hpp"
 #include "setup.hpp"
 
+#include <algorithm>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
@@ -448,6 +449,43 @@ int color_level(Level& L, hipStream_t s)
     return 0;
 }
 
+// Per colour and rows-per-thread, the widest slot window a sweep thread needs
+// (Level::win), from the row descriptors.
+static int window_stats(Level& L, hipStream_t s)
+{
+    std::vector<int32_t> h((size_t)L.m);
+    HPCGX_HIP(hipMemcpyAsync(h.data(), L.desc, sizeof(int32_t) * L.m, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+    for(int c = 0; c < L.ncolors; ++c)
+    {
+        for(int R = 1; R <= 2; ++R)
+        {
+            int wl = 0, wu = 0, wt = 0;
+            const int64_t b = L.offsets[c], e = L.offsets[c] + L.sizes[c];
+            for(int64_t r0 = b; r0 < e; r0 += R)
+            {
+                int maxd = 0, maxe = 0, mind1 = W + 1, mintl = W + 1;
+                for(int64_t r = r0; r < r0 + R && r < e; ++r)
+                {
+                    const int v = h[(size_t)r];
+                    const int d = v & 255, tl = (v >> 8) & 255, en = (v >> 16) & 255;
+                    maxd = std::max(maxd, d);
+                    maxe = std::max(maxe, en);
+                    mind1 = std::min(mind1, d + 1);
+                    mintl = std::min(mintl, tl);
+                }
+                wl = std::max(wl, maxd);
+                wu = std::max(wu, maxe - mind1);
+                wt = std::max(wt, maxe - mintl);
+            }
+            L.win[WIN_LOWER][R - 1][c] = (unsigned char)wl;
+            L.win[WIN_UPPER][R - 1][c] = (unsigned char)wu;
+            L.win[WIN_TAIL][R - 1][c] = (unsigned char)wt;
+        }
+    }
+    return 0;
+}
+
 int build_level(Level& L, int nx, int ny, int nz, bool with_f64, bool with_f16, hipStream_t s)
 {
     L.nx = nx;
@@ -488,7 +526,7 @@ int build_level(Level& L, int nx, int ny, int nz, bool with_f64, bool with_f16, 
     HPCGX_HIP(hipStreamSynchronize(s));
     HPCGX_HIP(hipFree(L.iperm_tmp));
     L.iperm_tmp = nullptr;
-    return 0;
+    return window_stats(L, s);
 }
 
 int build_transfer(Level& F, Level& C, hipStream_t s)
