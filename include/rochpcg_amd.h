@@ -179,10 +179,12 @@ int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps,
  * reference counterpart -- the reference hard-codes 1024/256/128-thread
  * blocks, ComputeSPMV.cpp:262, ComputeSYMGS.cpp:395): op 0 = SpMV,
  * 1 = zero-guess forward sweep, 2 = zero-guess backward sweep, 3 = forward
- * sweep, 4 = backward sweep, 5 = restriction; block in {64,128,256} threads,
- * rows in {1,2} consecutive rows per thread (restriction: 1).  level -1 /
- * op -1 apply to all; prec 0 = f64 kernels, 1 = mixed kernels.  Colour
- * blocks not divisible by `rows` run with 1. */
+ * sweep, 4 = backward sweep, 5 = restriction, 6-9 = ops 1-4 on their
+ * narrow colours (slot window of at most 16 slots; ops 1-4 then apply to
+ * the wide colours); block in {64,128,256} threads, rows in {1,2}
+ * consecutive rows per thread (restriction: 1).  level -1 / op -1 apply to
+ * all; prec 0 = f64 kernels, 1 = mixed kernels.  Colour blocks not
+ * divisible by `rows` run with 1. */
 int hpcgx_set_shape(hpcgx_problem* p, int prec, int level, int op, int block, int rows);
 int hpcgx_get_shape(const hpcgx_problem* p, int prec, int level, int op, int* block, int* rows);
 int hpcgx_synchronize(hpcgx_problem* p);

@@ -29,7 +29,8 @@ CG_MIXED = 1
  OP_RESTRICT) = range(9)
 PHASES = ("spmv", "mg", "ddot", "waxpby", "true_residual")
 # launch-shape operators (hpcgx_set_shape)
-SH_SPMV, SH_FWD_ZERO, SH_BWD_ZERO, SH_FWD_FULL, SH_BWD_FULL, SH_RESTRICT = range(6)
+(SH_SPMV, SH_FWD_ZERO, SH_BWD_ZERO, SH_FWD_FULL, SH_BWD_FULL, SH_RESTRICT,
+ SH_FWD_ZERO_N, SH_BWD_ZERO_N, SH_FWD_FULL_N, SH_BWD_FULL_N) = range(10)
 
 
 class HPCGError(RuntimeError):

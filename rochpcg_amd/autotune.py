@@ -1,7 +1,8 @@
 """Per-level launch-shape autotuner.
 
 For every level of a problem and every tunable operator (SpMV, the four
-SymGS sweep kernels, the fused restriction) and both precisions, time each
+SymGS sweep kernels on their wide and on their narrow colours, the fused
+restriction) and both precisions, time each
 compiled launch shape -- rows per thread {1, 2} x threads per block
 {64, 128, 256} -- with HIP events on the solver stream and keep the fastest.
 Results are cached as JSON keyed by the level's grid and the precision, so a
@@ -15,16 +16,21 @@ import json
 import os
 
 from . import (OP_BWD_FULL, OP_BWD_ZERO, OP_FWD_FULL, OP_FWD_ZERO, OP_RESTRICT, OP_SPMV,
-               SH_BWD_FULL, SH_BWD_ZERO, SH_FWD_FULL, SH_FWD_ZERO, SH_RESTRICT, SH_SPMV,
-               STORE_F64, STORE_MIXED)
+               SH_BWD_FULL, SH_BWD_FULL_N, SH_BWD_ZERO, SH_BWD_ZERO_N, SH_FWD_FULL, SH_FWD_FULL_N,
+               SH_FWD_ZERO, SH_FWD_ZERO_N, SH_RESTRICT, SH_SPMV, STORE_F64, STORE_MIXED)
 
 SHAPES = [(block, rows) for rows in (1, 2) for block in (64, 128, 256)]
-OPS = {  # tunable operator -> timing operator
+OPS = {  # tunable operator -> timing operator (a sweep's wide and narrow
+    # colours are timed together; the other set keeps its current shape)
     SH_SPMV: OP_SPMV, SH_FWD_ZERO: OP_FWD_ZERO, SH_BWD_ZERO: OP_BWD_ZERO,
     SH_FWD_FULL: OP_FWD_FULL, SH_BWD_FULL: OP_BWD_FULL, SH_RESTRICT: OP_RESTRICT,
+    SH_FWD_ZERO_N: OP_FWD_ZERO, SH_BWD_ZERO_N: OP_BWD_ZERO, SH_FWD_FULL_N: OP_FWD_FULL,
+    SH_BWD_FULL_N: OP_BWD_FULL,
 }
 OP_NAMES = {SH_SPMV: "spmv", SH_FWD_ZERO: "fwd_zero", SH_BWD_ZERO: "bwd_zero",
-            SH_FWD_FULL: "fwd", SH_BWD_FULL: "bwd", SH_RESTRICT: "restrict"}
+            SH_FWD_FULL: "fwd", SH_BWD_FULL: "bwd", SH_RESTRICT: "restrict",
+            SH_FWD_ZERO_N: "fwd_zero_narrow", SH_BWD_ZERO_N: "bwd_zero_narrow", SH_FWD_FULL_N: "fwd_narrow",
+            SH_BWD_FULL_N: "bwd_narrow"}
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "rochpcg_amd", "shapes.json")
 
 

@@ -67,7 +67,8 @@ enum WinKind : int
     WIN_COUNT = 3
 };
 
-// tunable operators of a level
+// tunable operators of a level.  A sweep has two shapes: one for its wide
+// colours (slot window over 16) and one (SH_NARROW + op) for the others.
 enum ShapeOp : int
 {
     SH_SPMV = 0,
@@ -76,8 +77,13 @@ enum ShapeOp : int
     SH_FWD_FULL = 3,
     SH_BWD_FULL = 4,
     SH_RESTRICT = 5,
-    SH_COUNT = 6
+    SH_FWD_ZERO_N = 6,
+    SH_BWD_ZERO_N = 7,
+    SH_FWD_FULL_N = 8,
+    SH_BWD_FULL_N = 9,
+    SH_COUNT = 10
 };
+constexpr int SH_NARROW = SH_FWD_ZERO_N - SH_FWD_ZERO;
 
 // One level of the hierarchy, device pointers plus host-side colour table.
 // Rows are in multicolour order: colour c owns rows [offsets[c], offsets[c+1]).

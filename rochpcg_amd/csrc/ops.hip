@@ -460,26 +460,31 @@ static void with_tail_window(int w, F&& f)
         f(integral_constant<int, W>{});
 }
 
-// one launch per colour c in [c0, c1) (step +1) or (c0, c1] (step -1)
+// one launch per colour c in [c0, c1) (step +1) or (c0, c1] (step -1); a
+// colour whose slot window (kind `wk`, one row per thread) exceeds 16 slots
+// runs with the op's wide shape, the others with its narrow shape
 template <typename Launch>
-static int per_color(const Level& L, int prec, int which, int c_first, int c_last, int step, Launch&& launch)
+static int per_color(const Level& L, int prec, int which, int wk, int c_first, int c_last, int step,
+                     Launch&& launch)
 {
-    const Shape sh = fit(L, L.shp[prec][which]);
-    return with_shape(sh, [&](auto RR, auto BB) -> int {
-        constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
-        for(int c = c_first; step > 0 ? c <= c_last : c >= c_last; c += step)
-        {
-            if(L.sizes[c] == 0) continue;
+    for(int c = c_first; step > 0 ? c <= c_last : c >= c_last; c += step)
+    {
+        if(L.sizes[c] == 0) continue;
+        const bool wide = (wk < 0) || L.win[wk][0][c] > 16;
+        const Shape sh = fit(L, L.shp[prec][wide ? which : which + SH_NARROW]);
+        HPCGX_TRY(with_shape(sh, [&](auto RR, auto BB) -> int {
+            constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             const dim3 grid(nblk(ceil_div(L.sizes[c], R), BS));
             // odd colours walk their blocks backwards (in both sweeps: forward
             // ends on colour 7 reversed, backward starts on colour 6 forwards,
             // so every launch starts on the slab the previous one ended on)
             const int64_t size = (c & 1) ? -(int64_t)L.sizes[c] : (int64_t)L.sizes[c];
-            launch(std::integral_constant<int, R>{}, std::integral_constant<int, BS>{}, grid, c, size);
+            launch(RR, BB, grid, c, size);
             HPCGX_KCHECK();
-        }
-        return 0;
-    });
+            return 0;
+        }));
+    }
+    return 0;
 }
 
 // The sweep kernels of a level: part 0 = zero-guess forward (colours 0..n-1),
@@ -500,7 +505,7 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
     switch(part)
     {
     case 0:
-        return per_color(L, pr, SH_FWD_ZERO, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
+        return per_color(L, pr, SH_FWD_ZERO, WIN_LOWER, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             with_window(L.win[WIN_LOWER][R - 1][c], [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
@@ -511,7 +516,7 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
     case 1:
     case 4:
         if(last < 1) return 0;
-        return per_color(L, pr, SH_BWD_ZERO, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
+        return per_color(L, pr, SH_BWD_ZERO, WIN_UPPER, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             with_window(L.win[WIN_UPPER][R - 1][c], [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
@@ -525,7 +530,7 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
         });
     case 2:
     case 5:
-        return per_color(L, pr, SH_FWD_FULL, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
+        return per_color(L, pr, SH_FWD_FULL, part == 5 ? WIN_LOWER : -1, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             if(c < last && part == 5)
             {
@@ -546,7 +551,7 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
         });
     case 3:
         if(last < 1) return 0;
-        return per_color(L, pr, SH_BWD_FULL, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
+        return per_color(L, pr, SH_BWD_FULL, WIN_UPPER, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             with_window(L.win[WIN_UPPER][R - 1][c], [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
