@@ -1,0 +1,33 @@
+// Extensions.hpp -- functions of the drop-in layer with no counterpart in the
+// reference.  The benchmark driver (rochpcg_amd/app/xhpcg.cpp) uses them for
+// what the reference does with its own GPU kernels or host code.
+#ifndef HPCGX_EXTENSIONS_HPP
+#define HPCGX_EXTENSIONS_HPP
+
+#include "CGData.hpp"
+#include "SparseMatrix.hpp"
+#include "Vector.hpp"
+
+// TestCG's exaggeration of the diagonal and of b (TestCG.cpp:67-95): the entry
+// at position i is scaled by (g+2)*1e6 when its global row id g < 9 and by 1e6
+// otherwise.  Like the reference, g is the natural row id of position i
+// (d_localToGlobalMap is not permuted), so on one rank positions 0..8 take the
+// large factors.
+int HPCGX_TestCGScale(const SparseMatrix& A, Vector& diagonal, Vector& b);
+
+// Runs `iterations` preconditioned FP64 CG iterations on (A, b) from x, one
+// launch at a time with HIP events around each phase, and adds the seconds
+// of each phase into times[] in the layout of the reference's CG timers
+// (CG.cpp:68-82): times[0] total, [1] dot products, [2] vector updates,
+// [3] SpMV, [5] MG.
+int HPCGX_CGPhaseTimes(const SparseMatrix& A, CGData& data, const Vector& b, Vector& x, int iterations,
+                       double* times);
+
+// Device bytes held by the solver of A's hierarchy (matrices, transfer maps,
+// vectors and scratch), after OptimizeProblem.
+double HPCGX_DeviceBytes(const SparseMatrix& A);
+
+// Name of the GPU the library runs on (a short string in buf).
+int HPCGX_DeviceName(char* buf, int len);
+
+#endif

@@ -9,6 +9,7 @@
 // asynchronous on that stream except the dot products, which return host
 // values.  Errors return non-zero (the reference exits in HIP_CHECK).
 #include "../../include/hpcg/hpcg.hpp"
+#include "../../include/hpcg/Extensions.hpp"
 
 #include <chrono>
 #include <cstdio>
@@ -505,5 +506,57 @@ int CG(const SparseMatrix& A, CGData& data, const Vector& b, Vector& x, const in
         printf("HIP Initial Residual = %le\n", normr0);
         for(int k = 1; k <= niters; ++k) printf("HIP Iteration = %d   Scaled Residual = %le\n", k, hist[k] / normr0);
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Extensions.hpp
+
+int HPCGX_TestCGScale(const SparseMatrix& A, Vector& diagonal, Vector& b)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l) || l != 0) return 1;
+    return launch_testcg_scale(P->L[0].m, diagonal.d_values, b.d_values, P->s) ? 1 : 0;
+}
+
+int HPCGX_CGPhaseTimes(const SparseMatrix& A, CGData& data, const Vector& b, Vector& x, int iterations,
+                       double* times)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l) || l != 0 || iterations < 1 || !times) return 1;
+    (void)data;
+    double ph[PH_COUNT] = {};
+    if(cg_begin(*P, CG_F64, true, b.d_values, x.d_values, iterations, 0.0) ||
+       cg_iterations_profiled(*P, iterations, ph))
+        return 1;
+    int ni;
+    double nr, nr0;
+    if(cg_end(*P, &ni, &nr, &nr0, nullptr, 0)) return 1;
+    const double s = 1e-3;
+    times[1] += ph[PH_DOT] * s;
+    times[2] += (ph[PH_WAXPBY] + ph[PH_REPLACE]) * s;
+    times[3] += ph[PH_SPMV] * s;
+    times[5] += ph[PH_MG] * s;
+    times[0] += (ph[PH_DOT] + ph[PH_WAXPBY] + ph[PH_REPLACE] + ph[PH_SPMV] + ph[PH_MG]) * s;
+    return 0;
+}
+
+double HPCGX_DeviceBytes(const SparseMatrix& A)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l)) return 0.0;
+    return (double)problem_device_bytes(*P);
+}
+
+int HPCGX_DeviceName(char* buf, int len)
+{
+    if(!buf || len < 1) return 1;
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if(hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1;
+    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
     return 0;
 }

@@ -467,6 +467,24 @@ int cg_solve(Problem& P, int mode, bool precond, const double* b, double* x, int
     return cg_end(P, niters, normr, normr0, history, hist_len);
 }
 
+int64_t problem_device_bytes(const Problem& P)
+{
+    int64_t b = 0;
+    for(int l = 0; l < P.nlevels; ++l)
+    {
+        const Level& L = P.L[l];
+        const int64_t m = L.m;
+        b += 4 * W * m + 4 * 4 * m;   // columns; diag_idx, desc, perm, colors
+        if(P.has_f64) b += 8 * W * m + 8 * m + 8 * m + (l > 0 ? 16 * m : 0);   // values, 1/a_ii, t, r, x
+        if(P.has_mixed) b += 2 * W * m + 4 * m + 4 * m + (l > 0 ? 8 * m : 0);
+        b += 3 * 4 * L.mc;   // transfer maps
+    }
+    const int64_t m0 = P.L[0].m;
+    b += 7 * 8 * m0 + (P.has_f64 ? 8 * m0 : 0) + (P.has_mixed ? 4 * m0 : 0);   // CG vectors, z
+    b += 8 * P.partial_cap + 8 * (P.hist_cap + 1) + (int64_t)sizeof(CGScalars) + 64;
+    return b;
+}
+
 double hpcg_flops_per_iter(const Problem& P)
 {
     // ReportResults.cpp:73-89, one iteration, 1 pre- and 1 post-smoother

@@ -60,3 +60,52 @@ def test_drop_in_verbose_log_format(hip):
     lines = [ln for ln in out.splitlines() if ln.startswith("HIP Iteration = ")]
     assert len(lines) == 50
     assert out.count("HIP Initial Residual = ") == 1
+
+
+XHPCG = os.path.join(ROOT, "rochpcg_amd", "xhpcg")
+
+
+def _yaml_fields(path):
+    """The driver's report as {section/key: value} (two-space indented YAML)."""
+    fields, stack = {}, []
+    with open(path) as f:
+        for line in f:
+            if not line.strip():
+                continue
+            depth = (len(line) - len(line.lstrip(" "))) // 2
+            key, _, val = line.strip().partition(":")
+            stack = stack[:depth] + [key]
+            fields["/".join(stack)] = val.strip()
+    return fields
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("extra", [[], ["--tol=1e-9"]])
+def test_benchmark_driver_report(tmp_path, extra):
+    """xhpcg runs main.cpp's phases on the drop-in layer and writes the HPCG
+    report: a valid 32^3 quick run, the V&V sections passed, 50 iterations per
+    set against its own 50-iteration tolerance (more against --tol=1e-9)."""
+    path = tmp_path / "report.yaml"
+    args = [XHPCG, "--nx=32", "--ny=32", "--nz=32", "--rt=0", "--quick", f"--yaml={path}"] + extra
+    print(f"[gpu] {' '.join(args)}", flush=True)
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    y = _yaml_fields(path)
+    assert y["Linear System Information/Number of Equations"] == str(32 ** 3)
+    assert y["Linear System Information/Number of Nonzero Terms"] == str(94 ** 3)
+    assert y["Reproducibility Information/Result"] == "PASSED"
+    assert float(y["Final Summary/HPCG result is VALID with a GFLOP/s rating of"]) > 0.0
+    opt = int(y["Iteration Count Information/Optimized CG iterations per set"])
+    if extra:
+        assert y["Spectral Convergence Tests/Result"].startswith("SKIPPED")
+        assert float(y["Iteration Count Information/Reference tolerance"]) == 1e-9
+        assert 50 < opt <= 100
+    else:
+        assert y["Spectral Convergence Tests/Result"] == "PASSED"
+        assert int(y["Spectral Convergence Tests/Unpreconditioned/Maximum iteration count"]) <= 12
+        assert int(y["Spectral Convergence Tests/Preconditioned/Maximum iteration count"]) <= 2
+        sym = "Departure from Symmetry |x'Ay-y'Ax|/(2*||x||*||A||*||y||)/epsilon"
+        assert y[f"{sym}/Result"] == "PASSED"
+        assert opt == 50
+    for k in ("DDOT", "WAXPBY", "SpMV", "MG", "Total"):
+        assert float(y[f"Benchmark Time Summary/{k}"]) > 0.0
