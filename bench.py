@@ -41,46 +41,6 @@ def profiled_traffic(dims):
     return None
 
 
-def level_rows(dims, levels):
-    out = []
-    d = list(dims)
-    for _ in range(levels):
-        out.append((d[0] * d[1] * d[2], (3 * d[0] - 2) * (3 * d[1] - 2) * (3 * d[2] - 2)))
-        d = [v // 2 for v in d]
-    return out
-
-
-def bytes_per_iteration(dims, levels, mode):
-    """Algorithmic HBM bytes of one CG iteration of this implementation:
-    every ELL slot read once per pass (27 x (4 B column + value)), vectors
-    read/written once, neighbour gathers assumed served from cache."""
-    rows = level_rows(dims, levels)
-    m0 = rows[0][0]
-    if mode == "f64":
-        vb, xb, rb = 8, 8, 8
-    else:
-        vb, xb, rb = 2, 4, 8
-    slot = 4 + vb
-    total = 0.0
-    for lv, (m, _) in enumerate(rows):
-        r_bytes = rb if lv == 0 else xb
-        zero = 27 * slot + r_bytes + xb + 7 / 8 * 2 * xb
-        if lv + 1 < levels:
-            restrict = (27 * slot + r_bytes + 8 + xb) / 8
-            prolong = (8 + 3 * xb) / 8
-            post = (27 * slot + r_bytes + xb + 7 / 8 * xb) + 7 / 8 * (4 + vb + 2 * xb + xb) + 13 * slot
-            total += m * (zero + restrict + prolong + post)
-        else:
-            total += m * zero
-    spmv = 27 * (4 + vb) + 16
-    zb = 8 if mode == "f64" else 4
-    vec = (8 + zb) + (zb + 16) + 48
-    total += m0 * (spmv + vec)
-    if mode == "mixed":
-        total += m0 * (spmv + 24 + 8) / 10.0   # true residual every 10 iterations
-    return total
-
-
 def spmv_bytes(m, vb=8):
     return m * (27 * (4 + vb) + 8 + 8)
 
@@ -231,8 +191,10 @@ def main():
     log(f"mixed: {K} iterations in {elmx * 1e3:.1f} ms")
     ms64 = el64 * 1e3 / K
     msmx = elmx * 1e3 / K
-    b64 = bytes_per_iteration(dims, 4, "f64")
-    bmx = bytes_per_iteration(dims, 4, "mixed")
+    # algorithmic bytes of this library's kernels (each ELL slot a kernel reads
+    # once, neighbour gathers from cache, own-row vectors once)
+    mg64, b64 = G.iteration_bytes(hip.CG_F64)
+    _, bmx = G.iteration_bytes(hip.CG_MIXED)
 
     # per-phase HIP-event timing of K FP64 iterations (eager launches)
     G.vec["x"].upload(np.zeros(G.m))
@@ -241,6 +203,8 @@ def main():
     G.cg_end()
     spmv_ms = ph["spmv"] / K
     spmv_gbps = spmv_bytes(G.m) / (spmv_ms * 1e-3) / 1e9
+    mg_ms = ph["mg"] / K
+    mg_gbps = mg64 / (mg_ms * 1e-3) / 1e9
     log(f"phases (ms per iteration): " + ", ".join(f"{k} {v / K:.3f}" for k, v in ph.items()))
 
     out = {
@@ -272,6 +236,10 @@ def main():
                      "unit": "GB/s", "frac": round(spmv_gbps / HBM_PEAK_GBPS, 4),
                      "traffic": profiled_traffic(dims), "bytes_per_launch": spmv_bytes(G.m),
                      "avg_launch_ms": round(spmv_ms, 4)},
+        "mg_roofline": {"kernels": "FP64 V-cycle, all levels (sweeps, restriction, prolongation)",
+                        "bound": "hbm", "achieved": round(mg_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": round(mg_gbps / HBM_PEAK_GBPS, 4), "bytes_per_vcycle": int(mg64),
+                        "ms_per_vcycle": round(mg_ms, 4)},
     }
 
     if not args.no_solve:

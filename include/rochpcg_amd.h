@@ -143,6 +143,12 @@ int hpcgx_set_residual_replacement(hpcgx_problem* p, int every);
 
 /* HPCG flop count of one CG iteration (ReportResults.cpp:73-89) */
 double hpcgx_flops_per_iteration(const hpcgx_problem* p);
+/* algorithmic HBM bytes of one V-cycle and of one CG iteration in `mode`
+ * (HPCGX_CG_F64 / HPCGX_CG_MIXED) as this library's kernels run them: each
+ * ELL slot a kernel reads counted once, neighbour gathers from cache, own-row
+ * vectors once (no reference counterpart; the reference's traffic model is
+ * ReportResults.cpp:107-138) */
+int hpcgx_iteration_bytes(const hpcgx_problem* p, int mode, double* vcycle_bytes, double* iteration_bytes);
 
 /* ---- batched solves (no reference counterpart) --------------------------- */
 

@@ -73,6 +73,7 @@ def _load():
         "hpcgx_cg_end": ([vp, P(i32), P(f64), P(f64), vp, i32], i32),
         "hpcgx_set_residual_replacement": ([vp, i32], i32),
         "hpcgx_flops_per_iteration": ([vp], f64),
+        "hpcgx_iteration_bytes": ([vp, i32, P(f64), P(f64)], i32),
         "hpcgx_time_operator": ([vp, i32, i32, i32, i32, P(f64)], i32),
         "hpcgx_set_shape": ([vp, i32, i32, i32, i32, i32], i32),
         "hpcgx_get_shape": ([vp, i32, i32, i32, P(i32), P(i32)], i32),
@@ -351,6 +352,12 @@ class Problem:
 
     def flops_per_iteration(self):
         return lib.hpcgx_flops_per_iteration(self.h)
+
+    def iteration_bytes(self, mode=CG_F64):
+        """Algorithmic HBM bytes (V-cycle, whole CG iteration) of this library's kernels."""
+        mg, it = C.c_double(), C.c_double()
+        _check(lib.hpcgx_iteration_bytes(self.h, mode, C.byref(mg), C.byref(it)))
+        return mg.value, it.value
 
     def time_operator(self, op, level=0, mixed=False, reps=10):
         ms = C.c_double()

@@ -341,6 +341,17 @@ double hpcgx_flops_per_iteration(const hpcgx_problem* p)
     return hpcg_flops_per_iter(*p->P);
 }
 
+int hpcgx_iteration_bytes(const hpcgx_problem* p, int mode, double* vcycle_bytes, double* iteration_bytes)
+{
+    CHECK_P(p);
+    if(mode != CG_F64 && mode != CG_MIXED)
+    {
+        set_error("invalid CG mode");
+        return -2;
+    }
+    return hpcgx::iteration_bytes(*p->P, mode, vcycle_bytes, iteration_bytes);
+}
+
 // ---------------------------------------------------------------------------
 int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps, double* avg_ms)
 {

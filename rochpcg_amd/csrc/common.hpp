@@ -119,6 +119,7 @@ struct Level
     // upper sums over the other colours
     bool u_ok = false;
     unsigned char win[WIN_COUNT][2][MAX_COLORS] = {};   // [kind][R-1][colour]
+    int64_t slots[WIN_COUNT][MAX_COLORS] = {};         // [kind][colour]: window slots summed over rows
 
     Shape shp[2][SH_COUNT];   // [precision][operator]
 };

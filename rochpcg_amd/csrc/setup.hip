@@ -477,6 +477,12 @@ static int window_stats(Level& L, hipStream_t s)
                 wl = std::max(wl, maxd);
                 wu = std::max(wu, maxe - mind1);
                 wt = std::max(wt, maxe - mintl);
+                if(R == 1)
+                {
+                    L.slots[WIN_LOWER][c] += maxd;
+                    L.slots[WIN_UPPER][c] += maxe - mind1;
+                    L.slots[WIN_TAIL][c] += maxe - mintl;
+                }
             }
             L.win[WIN_LOWER][R - 1][c] = (unsigned char)wl;
             L.win[WIN_UPPER][R - 1][c] = (unsigned char)wu;

@@ -467,6 +467,61 @@ int cg_solve(Problem& P, int mode, bool precond, const double* b, double* x, int
     return cg_end(P, niters, normr, normr0, history, hist_len);
 }
 
+// Algorithmic HBM bytes of one V-cycle from level l down, as this library's
+// kernels run it: every slot a kernel reads once (4 B column + value),
+// padding slots read only by the whole-row kernels, neighbour x gathers served
+// from cache, own-row vectors read and written once.
+static double vcycle_bytes(const Problem& P, int l, bool mixed)
+{
+    const Level& L = P.L[l];
+    const double ib = 4.0, vb = mixed ? 2.0 : 8.0, xb = mixed ? 4.0 : 8.0;
+    const double rb = (l == 0) ? 8.0 : xb;   // the CG residual is f64
+    const double slot = ib + vb;
+    const int last = L.ncolors - 1;
+    const bool coarsest = l + 1 >= P.nlevels;
+    const bool keep_u = !coarsest && L.u_ok;
+    double b = 0.0;
+    for(int c = 0; c <= last; ++c)
+    {
+        const double n = L.sizes[c];
+        b += L.slots[WIN_LOWER][c] * slot + n * (4 + vb + rb + xb);   // fwd_zero
+        if(c < last) b += L.slots[WIN_UPPER][c] * slot + n * (4 + vb + 2 * xb + (keep_u ? xb : 0));   // bwd_zero
+    }
+    if(coarsest) return b;
+    const double mc = (double)L.mc;
+    b += mc * (W * slot + rb + 4 + 4 + xb);   // fused residual + restriction (whole rows)
+    b += vcycle_bytes(P, l + 1, mixed);
+    b += mc * (4 + 4 + 3 * xb);                // prolongation
+    for(int c = 0; c <= last; ++c)
+    {
+        const double n = L.sizes[c];
+        if(c < last && keep_u)
+            b += (L.slots[WIN_LOWER][c] + L.slots[WIN_TAIL][c]) * slot + n * (4 + vb + rb + 3 * xb);
+        else
+            b += n * (W * slot + rb + xb + (c < last ? xb : 0));   // whole-row forward sweep
+        if(c < last) b += L.slots[WIN_UPPER][c] * slot + n * (4 + vb + 2 * xb);   // bwd_full
+    }
+    return b;
+}
+
+int iteration_bytes(const Problem& P, int mode, double* vcycle, double* iteration)
+{
+    const bool mixed = mode == CG_MIXED;
+    const double mg = vcycle_bytes(P, 0, mixed);
+    const double m0 = (double)P.L[0].m;
+    const double vb = mixed ? 2.0 : 8.0, zb = mixed ? 4.0 : 8.0;
+    double it = mg;
+    it += m0 * (W * (4 + vb) + 16);            // SpMV + p'Ap
+    it += m0 * (8 + zb);                       // r'z
+    it += m0 * (zb + 16);                      // p = z + beta p
+    it += m0 * 48;                             // x, r updates and ||r||
+    if(mixed && P.replace_every > 0)           // true residual every k iterations
+        it += m0 * (W * (4 + vb) + 16 + 24 + 16) / P.replace_every;
+    if(vcycle) *vcycle = mg;
+    if(iteration) *iteration = it;
+    return 0;
+}
+
 int64_t problem_device_bytes(const Problem& P)
 {
     int64_t b = 0;

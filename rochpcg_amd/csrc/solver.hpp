@@ -98,6 +98,8 @@ int cg_iterations_profiled(Problem& P, int count, double* phase_ms);
 int cg_end(Problem& P, int* niters, double* normr, double* normr0, double* history, int hist_len);
 // device bytes the problem holds (matrices, maps, vectors, scratch)
 int64_t problem_device_bytes(const Problem& P);
+// algorithmic HBM bytes of one V-cycle and of one CG iteration in `mode`
+int iteration_bytes(const Problem& P, int mode, double* vcycle, double* iteration);
 int cg_solve(Problem& P, int mode, bool precond, const double* b, double* x, int max_iter, double tol,
              bool use_graph, int* niters, double* normr, double* normr0, double* history, int hist_len);
 
