@@ -289,6 +289,7 @@ TestNormsResult test_norms(const std::vector<double>& v)
 
 int main(int argc, char** argv)
 {
+    setvbuf(stdout, nullptr, _IOLBF, 0);   // progress lines reach a pipe as they are printed
     const Params prm = parse(argc, argv);
     const int levels = 4;
     std::vector<double> times(10, 0.0);
@@ -592,7 +593,6 @@ int main(int argc, char** argv)
     HIPDeleteCGData(data);
     HIPDeleteVector(b);
     HIPDeleteVector(x);
-    HIPDeleteVector(xexact);
-    delete geom;
+    HIPDeleteVector(xexact);   // DeleteMatrix(A) released geom with the hierarchy
     return valid ? 0 : 2;
 }
