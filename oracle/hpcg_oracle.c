@@ -20,7 +20,10 @@
  *                        built in oracle/_ref.
  *   orc_f2c              closed form 2*ic per dimension; bijection onto the
  *                        all-even points.
- *   orc_row_hash         distinctness and class ordering among neighbours.
+ *   orc_row_hash         among neighbours, hash order == parity-class order
+ *                        (where the grid keeps neighbouring hashes apart);
+ *                        the JPL colouring it drives equals the closed-form
+ *                        class->colour table below.
  *   orc_jpl              closed-form colour table (class->colour
  *                        {7:0,6:1,1:2,2:3,5:4,4:5,3:6,0:7}) derived from the
  *                        glibc rand() draws 0,7,1,2,4,3,5,6; proper colouring

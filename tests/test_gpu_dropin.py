@@ -84,7 +84,8 @@ def _yaml_fields(path):
 def test_benchmark_driver_report(tmp_path, extra):
     """xhpcg runs main.cpp's phases on the drop-in layer and writes the HPCG
     report: a valid 32^3 quick run, the V&V sections passed, 50 iterations per
-    set against its own 50-iteration tolerance (more against --tol=1e-9)."""
+    set (a set is never shorter than the reference's 50; 32^3 reaches both its
+    own 50-iteration tolerance and 1e-9 within 50)."""
     path = tmp_path / "report.yaml"
     args = [XHPCG, "--nx=32", "--ny=32", "--nz=32", "--rt=0", "--quick", f"--yaml={path}"] + extra
     print(f"[gpu] {' '.join(args)}", flush=True)
@@ -99,7 +100,7 @@ def test_benchmark_driver_report(tmp_path, extra):
     if extra:
         assert y["Spectral Convergence Tests/Result"].startswith("SKIPPED")
         assert float(y["Iteration Count Information/Reference tolerance"]) == 1e-9
-        assert 50 < opt <= 100
+        assert opt == 50
     else:
         assert y["Spectral Convergence Tests/Result"] == "PASSED"
         assert int(y["Spectral Convergence Tests/Unpreconditioned/Maximum iteration count"]) <= 12
