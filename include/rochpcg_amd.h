@@ -188,8 +188,7 @@ int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps,
  * sweep, 4 = backward sweep, 5 = restriction, 6-9 = ops 1-4 on their
  * narrow colours (slot window of at most 16 slots; ops 1-4 then apply to
  * the wide colours); block in {64,128,256} threads, rows in {1,2}
- * consecutive rows per thread (restriction: 2 only where the injected rows
- * are contiguous, as they are for the parity colourings).  level -1 / op -1 apply to
+ * consecutive rows per thread (restriction: 1).  level -1 / op -1 apply to
  * all; prec 0 = f64 kernels, 1 = mixed kernels.  Colour blocks not
  * divisible by `rows` run with 1. */
 int hpcgx_set_shape(hpcgx_problem* p, int prec, int level, int op, int block, int rows);

@@ -97,7 +97,7 @@ def tune(prob, mixed=False, reps=3, rounds=3, levels=None, table=None, log=None)
         for op, top in OPS.items():
             if op == SH_RESTRICT and lv == last:
                 continue
-            cands = SHAPES
+            cands = SHAPES if op != SH_RESTRICT else [(b, 1) for b in (64, 128, 256)]
             samples = {c: [] for c in cands}
             for _ in range(rounds):
                 for block, rows in cands:

@@ -430,7 +430,7 @@ int hpcgx_set_shape(hpcgx_problem* p, int prec, int level, int op, int block, in
         {
             if(op >= 0 && o != op) continue;
             p->P->L[l].shp[prec][o].block = block;
-            p->P->L[l].shp[prec][o].rows = rows;
+            p->P->L[l].shp[prec][o].rows = (o == SH_RESTRICT) ? 1 : rows;
         }
     }
     p->P->invalidate_graphs();

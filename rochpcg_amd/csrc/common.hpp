@@ -118,7 +118,6 @@ struct Level
     // only last-colour rows, so the post-smoother may reuse the pre-smoother's
     // upper sums over the other colours
     bool u_ok = false;
-    int64_t inj_base = -1;   // >= 0: fine row of coarse natural row k is inj_base + k
     unsigned char win[WIN_COUNT][2][MAX_COLORS] = {};   // [kind][R-1][colour]
     int64_t slots[WIN_COUNT][MAX_COLORS] = {};         // [kind][colour]: window slots summed over rows
 

@@ -561,35 +561,4 @@ __device__ __forceinline__ void row_restrict(int64_t m, int64_t k, const int32_t
     rc[f2c_coarse[k]] = (CT)s;
 }
 
-// The same for injected rows that are contiguous (fine row base + k for
-// coarse natural row k, the whole last colour): R rows per thread with vector
-// loads, slots [0, e) of each row (all its entries), in the same order.
-template <typename VT, typename XT, typename RT, typename CT, int R>
-__device__ __forceinline__ void row_restrict_c(int64_t m, int64_t base, int64_t k, const int32_t* __restrict__ col,
-                                               const VT* __restrict__ val, const int32_t* __restrict__ desc,
-                                               const int32_t* __restrict__ f2c_coarse, const RT* __restrict__ r,
-                                               const XT* __restrict__ x, CT* __restrict__ rc)
-{
-    const int64_t row = base + k;
-    int32_t ds[R];
-    ld<R>(desc + row, ds);
-    int lo[R], hi[R];
-#pragma unroll
-    for(int j = 0; j < R; ++j)
-    {
-        lo[j] = 0;
-        hi[j] = desc_end(ds[j]);
-    }
-    int32_t cc[W][R];
-    VT av[W][R];
-    load_window<W, VT, R>(m, row, col, val, 0, lo, hi, cc, av);
-    XT xv[W][R];
-    gather<W, XT, R>(cc, x, xv);
-    XT s[R];
-    ld_cvt<R>(r + row, s);
-    sub_window<W, VT, XT, R>(cc, av, xv, s);
-#pragma unroll
-    for(int j = 0; j < R; ++j) rc[f2c_coarse[k + j]] = (CT)s[j];
-}
-
 }  // namespace hpcgx

@@ -132,18 +132,6 @@ __global__ __launch_bounds__(BS) void k_restrict(int64_t m, int64_t mc, const in
     row_restrict<VT, XT, RT, CT>(m, k, col, val, f2c_fine, f2c_coarse, r, x, rc);
 }
 
-template <typename VT, typename XT, typename RT, typename CT, int R, int BS>
-__global__ __launch_bounds__(BS) void k_restrict_c(int64_t m, int64_t base, int64_t mc, const int32_t* __restrict__ col,
-                                                   const VT* __restrict__ val, const int32_t* __restrict__ desc,
-                                                   const int32_t* __restrict__ f2c_coarse, const RT* __restrict__ r,
-                                                   const XT* __restrict__ x, CT* __restrict__ rc, const int* gate)
-{
-    if(gated(gate)) return;
-    const int64_t k = (xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x) * R;
-    if(k >= mc) return;
-    row_restrict_c<VT, XT, RT, CT, R>(m, base, k, col, val, desc, f2c_coarse, r, x, rc);
-}
-
 // Prolongation: xf[f2c_fine[k]] += xc[f2c_coarse[k]].
 template <typename XT, typename CT, int BS>
 __global__ __launch_bounds__(BS) void k_prolong(int64_t mc, const int32_t* __restrict__ f2c_fine,
@@ -558,14 +546,8 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
                 hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
                                    (int64_t)L.offsets[c], sz, L.ell_col, val, r, x, t, gate);
             else
-                // every neighbour of a last-colour row comes earlier, so its
-                // full forward step is the zero-guess one: x = (r - L x) / d,
-                // the same sum in the same order, reading the lower window only
-                with_window(L.win[WIN_LOWER][R - 1][c], [&](auto NN) {
-                    constexpr int NS = decltype(NN)::value;
-                    hipLaunchKernelGGL((k_fwd_zero<NS, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                       (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, r, x, gate);
-                });
+                hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, r, x, t, gate);
         });
     case 3:
         if(last < 1) return 0;
@@ -607,21 +589,7 @@ int launch_symgs(const Level& L, Prec prec, const void* r, bool r_f64, void* x, 
 template <typename VT, typename XT, typename RT>
 static int restrict_t(const Level& F, const void* r, const void* x, void* rc, hipStream_t s, const int* gate)
 {
-    const Shape sh = F.shp[std::is_same<VT, double>::value ? PREC_F64 : PREC_MIXED][SH_RESTRICT];
-    const int bs = sh.block;
-    if(F.inj_base >= 0)
-    {
-        // contiguous injected rows: R rows per thread when base and count allow
-        const int R = (sh.rows == 2 && F.inj_base % 2 == 0 && F.mc % 2 == 0 && F.m % 2 == 0) ? 2 : 1;
-        return with_shape(Shape{bs, R}, [&](auto RR, auto BB) -> int {
-            constexpr int RN = decltype(RR)::value, BS = decltype(BB)::value;
-            hipLaunchKernelGGL((k_restrict_c<VT, XT, RT, XT, RN, BS>), dim3(nblk(ceil_div(F.mc, RN), BS)), dim3(BS), 0,
-                               s, F.m, F.inj_base, F.mc, F.ell_col, vals_of<VT>(F), F.desc, F.f2c_coarse,
-                               (const RT*)r, (const XT*)x, (XT*)rc, gate);
-            HPCGX_KCHECK();
-            return 0;
-        });
-    }
+    const int bs = F.shp[std::is_same<VT, double>::value ? PREC_F64 : PREC_MIXED][SH_RESTRICT].block;
     auto go = [&](auto BB) -> int {
         constexpr int BS = decltype(BB)::value;
         hipLaunchKernelGGL((k_restrict<VT, XT, RT, XT, BS>), dim3(nblk(F.mc, BS)), dim3(BS), 0, s, F.m, F.mc,

@@ -553,9 +553,6 @@ int build_transfer(Level& F, Level& C, hipStream_t s)
     const int32_t last_off = F.offsets[F.ncolors - 1];
     F.u_ok = F.ncolors > 1;
     for(int32_t v : h) F.u_ok = F.u_ok && v >= last_off;
-    bool contig = !h.empty();
-    for(size_t k = 0; k < h.size() && contig; ++k) contig = (int64_t)h[k] == (int64_t)h[0] + (int64_t)k;
-    F.inj_base = contig ? h[0] : -1;
     return 0;
 }
 
