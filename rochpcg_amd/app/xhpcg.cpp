@@ -16,6 +16,8 @@
 //
 // Usage: xhpcg [nx ny nz [rt]] [--nx=N] [--ny=N] [--nz=N] [--rt=SECONDS]
 //              [--tol=T] [--quick] [--yaml=PATH]
+// Without dimensions it reads hpcg.dat from the working directory, like the
+// reference (ReadHpcgDat.cpp), and falls back to 16^3 and 0 s.
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -72,7 +74,7 @@ struct Yaml
 // ---------------------------------------------------------------------------
 struct Params
 {
-    int nx = 0, ny = 0, nz = 0, rt = -1;
+    int nx = 0, ny = 0, nz = 0, rt = 0;
     bool verify = true, quick = false;
     double tol = 0.0;
     std::string yaml;
@@ -80,13 +82,45 @@ struct Params
 
 bool starts(const char* s, const char* p) { return strncmp(s, p, strlen(p)) == 0; }
 
-// init.cpp:115-159: positional nx ny nz rt, "--key=" forms, --tol= turns the
-// reference CG off; a dimension under 16 takes the largest other one (or 16)
+void skip_line(FILE* f)
+{
+    int c;
+    while((c = fgetc(f)) != EOF && c != '\n') {}
+}
+
+// ReadHpcgDat.cpp:42-73: two header lines, "nx ny nz", "seconds", then the
+// optional process grid; dimensions under 16 become 16, a missing run time 1800 s.
+bool read_hpcg_dat(int* dims, int* rt, int* procs)
+{
+    FILE* f = fopen("hpcg.dat", "r");
+    if(!f) return false;
+    skip_line(f);
+    skip_line(f);
+    for(int i = 0; i < 3; ++i)
+        if(fscanf(f, "%d", dims + i) != 1 || dims[i] < 16) dims[i] = 16;
+    skip_line(f);
+    if(rt && (fscanf(f, "%d", rt) != 1 || *rt < 0)) *rt = 30 * 60;
+    skip_line(f);
+    for(int i = 0; i < 3; ++i)
+        if(fscanf(f, "%d", procs + i) != 1 || procs[i] < 1) procs[i] = 0;
+    fclose(f);
+    return true;
+}
+
+// init.cpp:115-159: the first arguments as nx ny nz rt (values under 10 are
+// ignored), "--key=" forms, --tol= turns the reference CG off; without
+// dimensions on the command line hpcg.dat supplies them; a dimension under 16
+// takes the largest other one, or 16.
 Params parse(int argc, char** argv)
 {
     Params P;
-    int pos[4] = {0, 0, 0, -1};
-    int npos = 0;
+    int pos[4] = {0, 0, 0, 0};
+    for(int i = 0; i < 4 && i + 1 < argc; ++i)
+        if(sscanf(argv[i + 1], "%d", pos + i) != 1 || pos[i] < 10) pos[i] = 0;
+    P.nx = pos[0];
+    P.ny = pos[1];
+    P.nz = pos[2];
+    int rt = pos[3];
     for(int i = 1; i < argc; ++i)
     {
         const char* a = argv[i];
@@ -97,7 +131,7 @@ Params parse(int argc, char** argv)
         else if(starts(a, "--nz="))
             P.nz = atoi(a + 5);
         else if(starts(a, "--rt="))
-            P.rt = atoi(a + 5);
+            rt = atoi(a + 5);
         else if(starts(a, "--tol="))
         {
             P.tol = atof(a + 6);
@@ -107,17 +141,22 @@ Params parse(int argc, char** argv)
             P.quick = true;
         else if(starts(a, "--yaml="))
             P.yaml = a + 7;
-        else if(a[0] != '-' && npos < 4)
-            pos[npos++] = atoi(a);
     }
     if(!P.nx && !P.ny && !P.nz)
     {
-        P.nx = pos[0];
-        P.ny = pos[1];
-        P.nz = pos[2];
+        int d[3] = {0, 0, 0}, procs[3] = {0, 0, 0}, file_rt = 0;
+        if(read_hpcg_dat(d, rt ? nullptr : &file_rt, procs))
+        {
+            P.nx = d[0];
+            P.ny = d[1];
+            P.nz = d[2];
+            if(!rt) rt = file_rt;
+            if(procs[0] * procs[1] * procs[2] > 1)
+                fprintf(stderr, "xhpcg: hpcg.dat asks for a %d x %d x %d process grid; this run uses one GPU\n",
+                        procs[0], procs[1], procs[2]);
+        }
     }
-    if(P.rt < 0) P.rt = (npos >= 4 && pos[3] >= 0) ? pos[3] : 60;
-    if(!P.nx && !P.ny && !P.nz) P.nx = P.ny = P.nz = 104;   // hpcg.dat's customary 104^3
+    P.rt = rt;
     int d[3] = {P.nx, P.ny, P.nz};
     for(int i = 0; i < 3; ++i)
     {

@@ -80,16 +80,22 @@ def _yaml_fields(path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("extra", [[], ["--tol=1e-9"]])
+@pytest.mark.parametrize("extra", [[], ["--tol=1e-9"], ["hpcg.dat"]])
 def test_benchmark_driver_report(tmp_path, extra):
     """xhpcg runs main.cpp's phases on the drop-in layer and writes the HPCG
     report: a valid 32^3 quick run, the V&V sections passed, 50 iterations per
     set (a set is never shorter than the reference's 50; 32^3 reaches both its
     own 50-iteration tolerance and 1e-9 within 50)."""
     path = tmp_path / "report.yaml"
-    args = [XHPCG, "--nx=32", "--ny=32", "--nz=32", "--rt=0", "--quick", f"--yaml={path}"] + extra
+    if extra == ["hpcg.dat"]:
+        # no dimensions on the command line: hpcg.dat in the working directory
+        (tmp_path / "hpcg.dat").write_text("HPCG benchmark input file\nfor the driver test\n32 32 32\n0\n")
+        extra = []
+        args = [XHPCG, "--quick", f"--yaml={path}"]
+    else:
+        args = [XHPCG, "--nx=32", "--ny=32", "--nz=32", "--rt=0", "--quick", f"--yaml={path}"] + extra
     print(f"[gpu] {' '.join(args)}", flush=True)
-    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert out.returncode == 0, out.stdout + out.stderr
     y = _yaml_fields(path)
     assert y["Linear System Information/Number of Equations"] == str(32 ** 3)
