@@ -437,27 +437,55 @@ int launch_spmv_f64x(const Level& L, Prec prec, const double* x, double* y, cons
     return spmv_t<__half, double, double>(L, x, y, ra, s, gate);
 }
 
-// compile-time slot-window bounds: 4, 16 or all 27 slots (tails: 8 or 27)
+// Compile-time slot-window bounds.  The bucket edges sit at the window sizes
+// of the 27-point operator's parity colourings (lower: 0 2 10 12 16 22 26,
+// upper: 4 10 14 16 24 26 per colour), so each colour's kernel keeps only the
+// registers its rows need; any larger window takes the next bucket up.
 template <typename F>
 static void with_window(int w, F&& f)
 {
     using std::integral_constant;
     if(w <= 4)
         f(integral_constant<int, 4>{});
+    else if(w <= 10)
+        f(integral_constant<int, 10>{});
+    else if(w <= 12)
+        f(integral_constant<int, 12>{});
     else if(w <= 16)
         f(integral_constant<int, 16>{});
+    else if(w <= 22)
+        f(integral_constant<int, 22>{});
+    else if(w <= 24)
+        f(integral_constant<int, 24>{});
     else
         f(integral_constant<int, W>{});
 }
 
+// (lower, last-colour) window pairs of the post-smoother's forward sweep, as
+// they occur per colour of the parity colourings; anything else takes the
+// smallest listed pair covering it, or (27, 27)
 template <typename F>
-static void with_tail_window(int w, F&& f)
+static void with_window_pair(int wl, int wt, F&& f)
 {
     using std::integral_constant;
-    if(w <= 8)
-        f(integral_constant<int, 8>{});
-    else
-        f(integral_constant<int, W>{});
+#define HPCGX_PAIR(A, B)                                                  \
+    if(wl <= (A) && wt <= (B))                                            \
+    {                                                                     \
+        f(integral_constant<int, A>{}, integral_constant<int, B>{});      \
+        return;                                                           \
+    }
+    HPCGX_PAIR(4, 4)
+    HPCGX_PAIR(4, 8)
+    HPCGX_PAIR(10, 2)
+    HPCGX_PAIR(12, 2)
+    HPCGX_PAIR(12, 4)
+    HPCGX_PAIR(16, 2)
+    HPCGX_PAIR(16, 4)
+    HPCGX_PAIR(22, 4)
+    HPCGX_PAIR(24, 4)
+    HPCGX_PAIR(W, 8)
+#undef HPCGX_PAIR
+    f(integral_constant<int, W>{}, integral_constant<int, W>{});
 }
 
 // one launch per colour c in [c0, c1) (step +1) or (c0, c1] (step -1); a
@@ -534,12 +562,10 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             if(c < last && part == 5)
             {
-                with_window(L.win[WIN_LOWER][R - 1][c], [&](auto NA_) {
-                    with_tail_window(L.win[WIN_TAIL][R - 1][c], [&](auto NB_) {
-                        constexpr int NA = decltype(NA_)::value, NB = decltype(NB_)::value;
-                        hipLaunchKernelGGL((k_fwd_full_u<NA, NB, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                           (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, r, x, t, gate);
-                    });
+                with_window_pair(L.win[WIN_LOWER][R - 1][c], L.win[WIN_TAIL][R - 1][c], [&](auto NA_, auto NB_) {
+                    constexpr int NA = decltype(NA_)::value, NB = decltype(NB_)::value;
+                    hipLaunchKernelGGL((k_fwd_full_u<NA, NB, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
+                                       (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, r, x, t, gate);
                 });
             }
             else if(c < last)
