@@ -356,10 +356,11 @@ __device__ __forceinline__ void row_fwd_zero(int64_t m, int64_t row, const int32
 }
 
 // Zero-initial-guess backward sweep: x = (x*d - U x) / d over the window
-// (d, e), NS >= max e - min(d+1).  With STORE_U it also keeps u = sum over
-// (d, tl) of a*x -- the upper entries outside the last colour, whose x the
-// coarse-grid correction leaves alone -- for the post-smoother's forward
-// sweep (row_fwd_full_u).
+// (d, e), NS >= max e - min(d+1).  With STORE_U it also keeps u = the sum
+// over (d, tl) of a*x -- the upper entries outside the last colour, whose x
+// the coarse-grid correction leaves alone -- for the post-smoother's forward
+// sweep (row_fwd_full_u); u is read off the running sum (x*d minus its value
+// at slot tl), so it costs no second accumulation.
 template <int NS, typename VT, typename XT, bool STORE_U, int R>
 __device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, const int32_t* __restrict__ desc,
@@ -385,23 +386,43 @@ __device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32
     for(int k = 0; k < R; ++k) s[k] = s[k] * d[k];
     XT xv[NS][R];
     gather<NS, XT, R>(cc, x, xv);
-    sub_window<NS, VT, XT, R>(cc, av, xv, s);
-#pragma unroll
-    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
-    st<R>(x + row, s);
-    if(STORE_U)
+    if constexpr(STORE_U)
     {
-        XT uu[R];
+        // u = x*d - (the running sum when it reaches the first last-colour
+        // slot): one snapshot register per row instead of a second chain
+        XT s0[R], snap[R];
+        bool taken[R];
 #pragma unroll
-        for(int k = 0; k < R; ++k) uu[k] = 0;
+        for(int k = 0; k < R; ++k)
+        {
+            s0[k] = s[k];
+            snap[k] = s[k];
+            taken[k] = false;
+        }
 #pragma unroll
         for(int q = 0; q < NS; ++q)
 #pragma unroll
             for(int k = 0; k < R; ++k)
-                if(cc[q][k] >= 0 && start + q < desc_tail(ds[k]))
-                    uu[k] = fmaT(cvt<XT>(av[q][k]), xv[q][k], uu[k]);
+            {
+                if(!taken[k] && start + q >= desc_tail(ds[k]))
+                {
+                    snap[k] = s[k];
+                    taken[k] = true;
+                }
+                if(cc[q][k] >= 0) s[k] = fmaT(-cvt<XT>(av[q][k]), xv[q][k], s[k]);
+            }
+        XT uu[R];
+#pragma unroll
+        for(int k = 0; k < R; ++k) uu[k] = s0[k] - (taken[k] ? snap[k] : s[k]);
         st<R>(u + row, uu);
     }
+    else
+    {
+        sub_window<NS, VT, XT, R>(cc, av, xv, s);
+    }
+#pragma unroll
+    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
+    st<R>(x + row, s);
 }
 
 // Full forward sweep over the whole row in ascending column order; keeps
