@@ -557,6 +557,7 @@ int HPCGX_DeviceName(char* buf, int len)
     hipDeviceProp_t prop;
     int dev = 0;
     if(hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1;
-    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    const char* name = prop.name[0] ? prop.name : "AMD GPU";
+    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", name, prop.gcnArchName, prop.multiProcessorCount);
     return 0;
 }
