@@ -138,7 +138,9 @@ int hpcgx_cg_iterate(hpcgx_problem* p, int count, int use_graph);
 int hpcgx_cg_iterate_profiled(hpcgx_problem* p, int count, double* phase_ms);
 int hpcgx_cg_end(hpcgx_problem* p, int* niters, double* normr, double* normr0, double* history,
                  int hist_len);
-/* mixed mode: recompute the true residual every k iterations (0 = never) */
+/* mixed mode: recompute the true residual every k iterations (0 = never, the
+ * default: the outer CG's operator is exact, 26 and -1 being exact in f16, so
+ * its recursive residual tracks the true one as in the FP64 CG) */
 int hpcgx_set_residual_replacement(hpcgx_problem* p, int every);
 
 /* HPCG flop count of one CG iteration (ReportResults.cpp:73-89) */

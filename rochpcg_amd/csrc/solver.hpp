@@ -57,7 +57,7 @@ struct Problem
     // CG run state
     int mode = CG_F64;
     bool precond = true;
-    int replace_every = 10;   // mixed mode: true residual every k iterations (0 = never)
+    int replace_every = 0;    // mixed mode: true residual every k iterations (0 = never)
     int host_iter = 0;        // iterations enqueued since cg_begin
     const double* cur_b = nullptr;
     double* cur_x = nullptr;

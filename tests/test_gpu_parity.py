@@ -278,6 +278,15 @@ def test_mixed_precision_reaches_1e9(hip, oracle_mod, dims):
     assert rtrue <= 2e-9
     assert mix["niters"] <= f64["niters"] + 3
     assert np.linalg.norm(xm - x64) / np.linalg.norm(x64) < 1e-7
+    # with the true residual recomputed every 7 iterations (graph and eager)
+    G.set_residual_replacement(7)
+    for use_graph in (True, False):
+        G.vec["x"].upload(np.zeros(G.m))
+        rep = G.cg(max_iter=200, tol=1e-9, mode=hip.CG_MIXED, use_graph=use_graph)
+        xr = G.vec["x"].host()
+        assert rep["normr"] / rep["normr0"] <= 1e-9
+        assert np.linalg.norm(b - O.spmv(xr)) / np.linalg.norm(b) <= 2e-9
+        assert abs(rep["niters"] - mix["niters"]) <= 2
     G.close()
 
 
