@@ -11,6 +11,14 @@ wherever oracle/_ref cannot be built.
                       multicolour order.
   kernels_16.npz      16x16x16 (JPL order): x ~ U[-1,1] (numpy default_rng(2024)),
                       A x, one SymGS from x = 0 and from x, one V-cycle.
+  ref_tolerance.npz   the reference tolerance of main.cpp:336-345 -- normr/normr0
+                      after 50 natural-order CG_ref iterations, b = A*ones,
+                      x0 = 0, 4 levels -- for the cube sides of the reference's
+                      check_niter table (test_rochpcg.cpp:88-119): 104, 256 and
+                      288 (16 and 32 reduce below 1e-20, into rounding noise)
+                      (written by `make_golden.py ref-tolerance`; 8 threads,
+                      oracle and reference within 1e-10 relative, the
+                      reference's value kept).
 """
 import os
 import sys
@@ -54,5 +62,30 @@ def main():
     print("golden fixtures written")
 
 
+def ref_tolerance(sizes=(104, 256, 288)):
+    oracle.build(ref=True)
+    oracle.set_threads(8)
+    out = {"n": np.array(sizes), "reduction": np.zeros(len(sizes))}
+    for i, n in enumerate(sizes):
+        P = oracle.Problem(n, nlevels=4, order=oracle.ORDER_NATURAL)
+        b = P.rhs_ones()
+        o = P.cg(b, max_iter=50)
+        R = oracle.RefHierarchy(P)
+        r = R.cg(b, max_iter=50)
+        R.close()
+        # 8 OpenMP threads: the two codes' dot reductions round differently,
+        # so they must agree to 1e-10 relative here, not bit for bit
+        ro, rr = o["normr"] / o["normr0"], r["normr"] / r["normr0"]
+        if not abs(ro - rr) <= 1e-10 * rr:
+            raise SystemExit(f"oracle and reference disagree on the reference tolerance at {n}: {ro} vs {rr}")
+        out["reduction"][i] = rr
+        print(f"n={n}: 50-iteration reduction {rr:.6e} (oracle {ro:.6e})", flush=True)
+        del P
+    np.savez(os.path.join(HERE, "ref_tolerance.npz"), **out)
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "ref-tolerance":
+        ref_tolerance()
+    else:
+        main()

@@ -351,3 +351,22 @@ def test_iteration_byte_model(hip):
     nnz = sum(G.levels[lv].nnz for lv in range(4))
     assert mg64 >= 12 * nnz
     G.close()
+
+
+@pytest.mark.parametrize("n", [104, 256, 288])
+def test_check_niter_against_reference_tolerance(hip, n):
+    """The reference's acceptance check (test_rochpcg.cpp:88-119, 323-324): the
+    multicolour CG reaches the natural-order CG_ref's 50-iteration residual
+    reduction (tests/golden/ref_tolerance.npz, computed by the reference's
+    own _ref code) within check_niter(n) iterations."""
+    import os
+    from rochpcg_amd import vv
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_tolerance.npz"))
+    tol = float(g["reduction"][list(g["n"]).index(n)])
+    G = _hip_problem(hip, (n, n, n), stores=hip.STORE_F64)
+    res = G.cg(max_iter=500, tol=tol, history=False)
+    print(f"[gpu] {n}^3: reference tolerance {tol:.3e} reached in {res['niters']} iterations "
+          f"(check_niter {vv.check_niter(n)})", flush=True)
+    assert res["normr"] / res["normr0"] <= tol
+    assert res["niters"] <= vv.check_niter(n)
+    G.close()
