@@ -68,25 +68,39 @@ __global__ __launch_bounds__(BS) void kb_spmv(const BSeg* __restrict__ segs, int
     if(DOT) block_partial<BS>(acc, S.ra.partial, lb);
 }
 
-template <typename VT, typename XT, typename RT, int BS>
+// The sweeps take the slot-window bounds of the launch (the widest window of
+// the colour over all its problems, Level::win), like the single-problem
+// kernels.
+template <int NS, typename VT, typename XT, typename RT, int BS>
 __global__ __launch_bounds__(BS) void kb_fwd_zero(const BSeg* __restrict__ segs, int nseg)
 {
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    row_fwd_zero<W, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x);
+    row_fwd_zero<NS, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x);
 }
 
-template <typename VT, typename XT, int BS>
+template <int NS, typename VT, typename XT, int BS>
 __global__ __launch_bounds__(BS) void kb_bwd_zero(const BSeg* __restrict__ segs, int nseg)
 {
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
     if(S.t)
-        row_bwd_zero<W, VT, XT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, (XT*)S.t);
+        row_bwd_zero<NS, VT, XT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, (XT*)S.t);
     else
-        row_bwd_zero<W, VT, XT, false, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, nullptr);
+        row_bwd_zero<NS, VT, XT, false, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, nullptr);
+}
+
+// every segment reuses its pre-smoother's upper sums (BLaunch::all_u)
+template <int NA, int NB, typename VT, typename XT, typename RT, int BS>
+__global__ __launch_bounds__(BS) void kb_fwd_full_u(const BSeg* __restrict__ segs, int nseg)
+{
+    BSEG();
+    const int64_t i = lb * BS + threadIdx.x;
+    if(i >= S.size) return;
+    row_fwd_full_u<NA, NB, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x,
+                                          (XT*)S.t);
 }
 
 template <typename VT, typename XT, typename RT, int BS>
@@ -97,7 +111,7 @@ __global__ __launch_bounds__(BS) void kb_fwd_full(const BSeg* __restrict__ segs,
     if(i >= S.size) return;
     if(S.use_u)
         row_fwd_full_u<W, W, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x,
-                                      (XT*)S.t);
+                                            (XT*)S.t);
     else if(S.t)
         row_fwd_full<VT, XT, RT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
                                           (XT*)S.t);
@@ -106,13 +120,13 @@ __global__ __launch_bounds__(BS) void kb_fwd_full(const BSeg* __restrict__ segs,
                                            nullptr);
 }
 
-template <typename VT, typename XT, int BS>
+template <int NS, typename VT, typename XT, int BS>
 __global__ __launch_bounds__(BS) void kb_bwd_full(const BSeg* __restrict__ segs, int nseg)
 {
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    row_bwd_full<W, VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const XT*)S.t, (XT*)S.x);
+    row_bwd_full<NS, VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const XT*)S.t, (XT*)S.x);
 }
 
 template <typename VT, typename XT, typename RT, int BS>
@@ -241,21 +255,37 @@ static void launch_bs(const BLaunch& L, hipStream_t s)
         }
         break;
     case BK_FWD_ZERO:
-        if(f64)
-            hipLaunchKernelGGL((kb_fwd_zero<double, double, double, BS>), g, b, 0, s, T, n);
-        else if(L.r_f64)
-            hipLaunchKernelGGL((kb_fwd_zero<__half, float, double, BS>), g, b, 0, s, T, n);
-        else
-            hipLaunchKernelGGL((kb_fwd_zero<__half, float, float, BS>), g, b, 0, s, T, n);
+        with_window(L.win, [&](auto NN) {
+            constexpr int NS = decltype(NN)::value;
+            if(f64)
+                hipLaunchKernelGGL((kb_fwd_zero<NS, double, double, double, BS>), g, b, 0, s, T, n);
+            else if(L.r_f64)
+                hipLaunchKernelGGL((kb_fwd_zero<NS, __half, float, double, BS>), g, b, 0, s, T, n);
+            else
+                hipLaunchKernelGGL((kb_fwd_zero<NS, __half, float, float, BS>), g, b, 0, s, T, n);
+        });
         break;
     case BK_BWD_ZERO:
-        if(f64)
-            hipLaunchKernelGGL((kb_bwd_zero<double, double, BS>), g, b, 0, s, T, n);
-        else
-            hipLaunchKernelGGL((kb_bwd_zero<__half, float, BS>), g, b, 0, s, T, n);
+        with_window(L.win, [&](auto NN) {
+            constexpr int NS = decltype(NN)::value;
+            if(f64)
+                hipLaunchKernelGGL((kb_bwd_zero<NS, double, double, BS>), g, b, 0, s, T, n);
+            else
+                hipLaunchKernelGGL((kb_bwd_zero<NS, __half, float, BS>), g, b, 0, s, T, n);
+        });
         break;
     case BK_FWD_FULL:
-        if(f64)
+        if(L.all_u)
+            with_window_pair(L.win, L.win_tail, [&](auto NA_, auto NB_) {
+                constexpr int NA = decltype(NA_)::value, NB = decltype(NB_)::value;
+                if(f64)
+                    hipLaunchKernelGGL((kb_fwd_full_u<NA, NB, double, double, double, BS>), g, b, 0, s, T, n);
+                else if(L.r_f64)
+                    hipLaunchKernelGGL((kb_fwd_full_u<NA, NB, __half, float, double, BS>), g, b, 0, s, T, n);
+                else
+                    hipLaunchKernelGGL((kb_fwd_full_u<NA, NB, __half, float, float, BS>), g, b, 0, s, T, n);
+            });
+        else if(f64)
             hipLaunchKernelGGL((kb_fwd_full<double, double, double, BS>), g, b, 0, s, T, n);
         else if(L.r_f64)
             hipLaunchKernelGGL((kb_fwd_full<__half, float, double, BS>), g, b, 0, s, T, n);
@@ -263,10 +293,13 @@ static void launch_bs(const BLaunch& L, hipStream_t s)
             hipLaunchKernelGGL((kb_fwd_full<__half, float, float, BS>), g, b, 0, s, T, n);
         break;
     case BK_BWD_FULL:
-        if(f64)
-            hipLaunchKernelGGL((kb_bwd_full<double, double, BS>), g, b, 0, s, T, n);
-        else
-            hipLaunchKernelGGL((kb_bwd_full<__half, float, BS>), g, b, 0, s, T, n);
+        with_window(L.win, [&](auto NN) {
+            constexpr int NS = decltype(NN)::value;
+            if(f64)
+                hipLaunchKernelGGL((kb_bwd_full<NS, double, double, BS>), g, b, 0, s, T, n);
+            else
+                hipLaunchKernelGGL((kb_bwd_full<NS, __half, float, BS>), g, b, 0, s, T, n);
+        });
         break;
     case BK_RESTRICT:
         if(f64)
@@ -327,6 +360,18 @@ struct Planner
     std::vector<BLaunch>* out;
     int prec;
     int next_bf32 = 0;   // the next vector launch reads an f32 z
+    // slot windows of the next sweep launch (full rows unless set)
+    int next_win = W, next_tail = W, next_all_u = 0;
+
+    // widest window of kind `wk` of colour c on level l over the problems
+    // that have the colour
+    int widest(int l, int wk, int c)
+    {
+        int w = 0;
+        for(Problem* P : B.probs)
+            if(c < P->L[l].ncolors) w = std::max<int>(w, P->L[l].win[wk][0][c]);
+        return w;
+    }
 
     // append a launch whose segments are built by `fill(k, seg) -> bool`
     template <typename Fill>
@@ -369,7 +414,12 @@ struct Planner
         L.nblocks = nb;
         L.beta = beta;
         L.bf32 = next_bf32;
+        L.win = next_win;
+        L.win_tail = next_tail;
+        L.all_u = next_all_u;
         next_bf32 = 0;
+        next_win = next_tail = W;
+        next_all_u = 0;
         if(!segs.empty())
         {
             BSeg* d = nullptr;
@@ -428,15 +478,21 @@ struct Planner
         };
         const int C = max_colors(l);
         for(int c = 0; c < C; ++c)
+        {
+            next_win = widest(l, WIN_LOWER, c);
             HPCGX_TRY(add(BK_FWD_ZERO, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) { return level_seg(k, s, c); }));
+        }
         // the pre-smoother keeps its upper sums for the post-smoother where
         // the correction changes only last-colour rows (Level::u_ok)
         for(int c = C - 2; c >= 0; --c)
+        {
+            next_win = widest(l, WIN_UPPER, c);
             HPCGX_TRY(add(BK_BWD_ZERO, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
                 if(!(c <= B.probs[k]->L[l].ncolors - 2 && level_seg(k, s, c))) return false;
                 if(!last && B.probs[k]->L[l].u_ok) s.t = vecs(B.probs[k])[l].t;
                 return true;
             }));
+        }
         if(last) return 0;
         HPCGX_TRY(add(BK_RESTRICT, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
             Problem* P = B.probs[k];
@@ -464,6 +520,16 @@ struct Planner
             return true;
         }));
         for(int c = 0; c < C; ++c)
+        {
+            bool all_u = true;
+            for(Problem* P : B.probs)
+                if(c < P->L[l].ncolors) all_u = all_u && c < P->L[l].ncolors - 1 && P->L[l].u_ok;
+            if(all_u)
+            {
+                next_all_u = 1;
+                next_win = widest(l, WIN_LOWER, c);
+                next_tail = widest(l, WIN_TAIL, c);
+            }
             HPCGX_TRY(add(BK_FWD_FULL, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
                 if(!level_seg(k, s, c)) return false;
                 const bool before_last = c < B.probs[k]->L[l].ncolors - 1;
@@ -471,12 +537,16 @@ struct Planner
                 s.use_u = before_last && B.probs[k]->L[l].u_ok;
                 return true;
             }));
+        }
         for(int c = C - 2; c >= 0; --c)
+        {
+            next_win = widest(l, WIN_UPPER, c);
             HPCGX_TRY(add(BK_BWD_FULL, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
                 if(!(c <= B.probs[k]->L[l].ncolors - 2 && level_seg(k, s, c))) return false;
                 s.t = vecs(B.probs[k])[l].t;
                 return true;
             }));
+        }
         return 0;
     }
 

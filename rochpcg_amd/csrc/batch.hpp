@@ -59,6 +59,9 @@ struct BLaunch
     int dot;          // SpMV: fuse p'Ap
     int bf32;         // vector kernels: the second operand (z) is f32
     int block;        // threads per block
+    int win;          // sweeps: widest slot window of the launch's rows (Level::win, one row per thread)
+    int win_tail;     // post-smoother forward sweep: widest last-colour window
+    int all_u;        // post-smoother forward sweep: every segment reuses its upper sums
     int nseg;
     int64_t nblocks;
     BSeg* segs;       // device table

@@ -582,4 +582,59 @@ __device__ __forceinline__ void row_restrict(int64_t m, int64_t k, const int32_t
     rc[f2c_coarse[k]] = (CT)s;
 }
 
+// ---------------------------------------------------------------------------
+// host side: dispatch of the slot-window bounds (shared by ops.hip and
+// batch.hip)
+
+// Compile-time slot-window bounds.  The bucket edges sit at the window sizes
+// of the 27-point operator's parity colourings (lower: 0 2 10 12 16 22 26,
+// upper: 4 10 14 16 24 26 per colour), so each colour's kernel keeps only the
+// registers its rows need; any larger window takes the next bucket up.
+template <typename F>
+static void with_window(int w, F&& f)
+{
+    using std::integral_constant;
+    if(w <= 4)
+        f(integral_constant<int, 4>{});
+    else if(w <= 10)
+        f(integral_constant<int, 10>{});
+    else if(w <= 12)
+        f(integral_constant<int, 12>{});
+    else if(w <= 16)
+        f(integral_constant<int, 16>{});
+    else if(w <= 22)
+        f(integral_constant<int, 22>{});
+    else if(w <= 24)
+        f(integral_constant<int, 24>{});
+    else
+        f(integral_constant<int, W>{});
+}
+
+// (lower, last-colour) window pairs of the post-smoother's forward sweep, as
+// they occur per colour of the parity colourings; anything else takes the
+// smallest listed pair covering it, or (27, 27)
+template <typename F>
+static void with_window_pair(int wl, int wt, F&& f)
+{
+    using std::integral_constant;
+#define HPCGX_PAIR(A, B)                                                  \
+    if(wl <= (A) && wt <= (B))                                            \
+    {                                                                     \
+        f(integral_constant<int, A>{}, integral_constant<int, B>{});      \
+        return;                                                           \
+    }
+    HPCGX_PAIR(4, 4)
+    HPCGX_PAIR(4, 8)
+    HPCGX_PAIR(10, 2)
+    HPCGX_PAIR(12, 2)
+    HPCGX_PAIR(12, 4)
+    HPCGX_PAIR(16, 2)
+    HPCGX_PAIR(16, 4)
+    HPCGX_PAIR(22, 4)
+    HPCGX_PAIR(24, 4)
+    HPCGX_PAIR(W, 8)
+#undef HPCGX_PAIR
+    f(integral_constant<int, W>{}, integral_constant<int, W>{});
+}
+
 }  // namespace hpcgx

@@ -437,57 +437,6 @@ int launch_spmv_f64x(const Level& L, Prec prec, const double* x, double* y, cons
     return spmv_t<__half, double, double>(L, x, y, ra, s, gate);
 }
 
-// Compile-time slot-window bounds.  The bucket edges sit at the window sizes
-// of the 27-point operator's parity colourings (lower: 0 2 10 12 16 22 26,
-// upper: 4 10 14 16 24 26 per colour), so each colour's kernel keeps only the
-// registers its rows need; any larger window takes the next bucket up.
-template <typename F>
-static void with_window(int w, F&& f)
-{
-    using std::integral_constant;
-    if(w <= 4)
-        f(integral_constant<int, 4>{});
-    else if(w <= 10)
-        f(integral_constant<int, 10>{});
-    else if(w <= 12)
-        f(integral_constant<int, 12>{});
-    else if(w <= 16)
-        f(integral_constant<int, 16>{});
-    else if(w <= 22)
-        f(integral_constant<int, 22>{});
-    else if(w <= 24)
-        f(integral_constant<int, 24>{});
-    else
-        f(integral_constant<int, W>{});
-}
-
-// (lower, last-colour) window pairs of the post-smoother's forward sweep, as
-// they occur per colour of the parity colourings; anything else takes the
-// smallest listed pair covering it, or (27, 27)
-template <typename F>
-static void with_window_pair(int wl, int wt, F&& f)
-{
-    using std::integral_constant;
-#define HPCGX_PAIR(A, B)                                                  \
-    if(wl <= (A) && wt <= (B))                                            \
-    {                                                                     \
-        f(integral_constant<int, A>{}, integral_constant<int, B>{});      \
-        return;                                                           \
-    }
-    HPCGX_PAIR(4, 4)
-    HPCGX_PAIR(4, 8)
-    HPCGX_PAIR(10, 2)
-    HPCGX_PAIR(12, 2)
-    HPCGX_PAIR(12, 4)
-    HPCGX_PAIR(16, 2)
-    HPCGX_PAIR(16, 4)
-    HPCGX_PAIR(22, 4)
-    HPCGX_PAIR(24, 4)
-    HPCGX_PAIR(W, 8)
-#undef HPCGX_PAIR
-    f(integral_constant<int, W>{}, integral_constant<int, W>{});
-}
-
 // one launch per colour c in [c0, c1) (step +1) or (c0, c1] (step -1); a
 // colour whose slot window (kind `wk`, one row per thread) exceeds 16 slots
 // runs with the op's wide shape, the others with its narrow shape
