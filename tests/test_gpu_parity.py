@@ -370,3 +370,22 @@ def test_check_niter_against_reference_tolerance(hip, n):
     assert res["normr"] / res["normr0"] <= tol
     assert res["niters"] <= vv.check_niter(n)
     G.close()
+
+
+@pytest.mark.parametrize("nlevels", [1, 2, 3])
+def test_fewer_levels_match_oracle(hip, oracle_mod, nlevels):
+    """Hierarchies of 1-3 levels (the coarsest level runs one zero-guess SymGS):
+    V-cycle and 30-iteration CG history against the oracle."""
+    dims = (32, 24, 16)
+    G = _hip_problem(hip, dims, nlevels=nlevels, stores=hip.STORE_F64)
+    O = oracle_mod.Problem(*dims, nlevels=nlevels, order=oracle_mod.ORDER_JPL)
+    rng = np.random.default_rng(5)
+    r = rng.uniform(-1, 1, G.m)
+    dr, dz = hip.DeviceArray.from_host(r), hip.DeviceArray(G.m)
+    G.mg(dr, dz)
+    assert _rel(dz.host(), O.mg(r)) <= 1e-12
+    res = G.cg(max_iter=30, tol=0.0, use_graph=True)
+    ref = O.cg(O.rhs_ones(), max_iter=30)
+    assert res["niters"] == 30
+    assert_history_close(res["history"], ref["history"])
+    G.close()
