@@ -8,6 +8,13 @@
 #include "SparseMatrix.hpp"
 #include "Vector.hpp"
 
+// Selects GPU `device` and creates the layer's stream and reduction scratch,
+// what the reference does in HPCG_Init before its setup timer starts
+// (init.cpp:193-233: device check, hipSetDevice, stream creation, workspace).
+// Returns non-zero for a device id the machine does not have.  Without this
+// call the layer starts on the current device at its first use.
+int HPCGX_Init(int device);
+
 // TestCG's exaggeration of the diagonal and of b (TestCG.cpp:67-95): the entry
 // at position i is scaled by (g+2)*1e6 when its global row id g < 9 and by 1e6
 // otherwise.  Like the reference, g is the natural row id of position i

@@ -74,7 +74,7 @@ struct Yaml
 // ---------------------------------------------------------------------------
 struct Params
 {
-    int nx = 0, ny = 0, nz = 0, rt = 0;
+    int nx = 0, ny = 0, nz = 0, rt = 0, device = 0;
     bool verify = true, quick = false;
     double tol = 0.0;
     std::string yaml;
@@ -108,7 +108,7 @@ bool read_hpcg_dat(int* dims, int* rt, int* procs)
 }
 
 // init.cpp:115-159: the first arguments as nx ny nz rt (values under 10 are
-// ignored), "--key=" forms, --tol= turns the reference CG off; without
+// ignored), "--key=" forms (--dev= picks the GPU), --tol= turns the reference CG off; without
 // dimensions on the command line hpcg.dat supplies them; a dimension under 16
 // takes the largest other one, or 16.
 Params parse(int argc, char** argv)
@@ -132,6 +132,8 @@ Params parse(int argc, char** argv)
             P.nz = atoi(a + 5);
         else if(starts(a, "--rt="))
             rt = atoi(a + 5);
+        else if(starts(a, "--dev="))
+            P.device = atoi(a + 6);
         else if(starts(a, "--tol="))
         {
             P.tol = atof(a + 6);
@@ -339,6 +341,9 @@ int main(int argc, char** argv)
         fprintf(stderr, "xhpcg: nx, ny and nz must be multiples of 8 for %d multigrid levels\n", levels);
         return 1;
     }
+
+    // device, stream and scratch before the setup timer, as HPCG_Init does
+    if(HPCGX_Init(prm.device)) return 1;
 
     // ---- setup (main.cpp:183-213) --------------------------------------
     double t = now();

@@ -31,15 +31,22 @@ struct Context
     double* partial = nullptr;
     double* dout = nullptr;
     double h_out = 0.0;
+    int device = -1;
     bool ok = false;
 };
 
-Context& ctx()
+Context& ctx_state()
 {
     static Context c;
+    return c;
+}
+
+Context& ctx()
+{
+    Context& c = ctx_state();
     if(!c.ok)
     {
-        if(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking) != hipSuccess ||
+        if(hipGetDevice(&c.device) != hipSuccess || hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking) != hipSuccess ||
            hipMalloc(&c.partial, sizeof(double) * FOLD_MAX) != hipSuccess ||
            hipMalloc(&c.dout, 8 * sizeof(double)) != hipSuccess)
         {
@@ -283,6 +290,8 @@ void GenerateProblem(SparseMatrix& A, Vector* b, Vector* x, Vector* xexact)
     if(xexact) HIPInitializeVector(*xexact, m);
     (void)natural_rhs(g.nx, g.ny, g.nz, b ? b->d_values : nullptr, xexact ? xexact->d_values : nullptr,
                       x ? x->d_values : nullptr, ctx().s);
+    // complete on return, like the reference's (GenerateProblem.cpp:462 reads a count back)
+    (void)hipStreamSynchronize(ctx().s);
 }
 
 void SetupHalo(SparseMatrix& A) { (void)A; }
@@ -300,6 +309,7 @@ void GenerateCoarseProblem(const SparseMatrix& Af)
     local_int_t* f2c = (local_int_t*)dalloc(sizeof(local_int_t) * (size_t)mc);
     local_int_t* c2f = (local_int_t*)dalloc(sizeof(local_int_t) * (size_t)Af.localNumberOfRows);
     (void)natural_transfer(gf.nx, gf.ny, gf.nz, f2c, c2f, ctx().s);
+    (void)hipStreamSynchronize(ctx().s);
     Vector* rc = new Vector;
     Vector* xc = new Vector;
     Vector* Axf = new Vector;
@@ -549,6 +559,23 @@ double HPCGX_DeviceBytes(const SparseMatrix& A)
     int l;
     if(!level_of(A, P, l)) return 0.0;
     return (double)problem_device_bytes(*P);
+}
+
+int HPCGX_Init(int device)
+{
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+    {
+        fprintf(stderr, "HPCGX_Init: invalid device ID %d (%d devices)\n", device, ndev);
+        return 1;
+    }
+    if(ctx_state().ok && ctx_state().device != device)
+    {
+        fprintf(stderr, "HPCGX_Init: the layer already runs on another device\n");
+        return 1;
+    }
+    if(hipSetDevice(device) != hipSuccess) return 1;
+    return ctx().ok ? 0 : 1;
 }
 
 int HPCGX_DeviceName(char* buf, int len)

@@ -449,46 +449,97 @@ int color_level(Level& L, hipStream_t s)
     return 0;
 }
 
-// Per colour and rows-per-thread, the widest slot window a sweep thread needs
-// (Level::win), from the row descriptors.
-static int window_stats(Level& L, hipStream_t s)
+// Window statistics of one colour's rows [off, off+size): maxima of the three
+// windows for one row per thread and for row pairs (off+2j, off+2j+1), and the
+// window sums for one row per thread.  stats[0..5] (int max), sums[0..2].
+__global__ __launch_bounds__(256) void k_window_stats(int64_t off, int64_t size, const int32_t* __restrict__ desc,
+                                                      int* __restrict__ stats, unsigned long long* __restrict__ sums)
 {
-    std::vector<int32_t> h((size_t)L.m);
-    HPCGX_HIP(hipMemcpyAsync(h.data(), L.desc, sizeof(int32_t) * L.m, hipMemcpyDeviceToHost, s));
-    HPCGX_HIP(hipStreamSynchronize(s));
-    for(int c = 0; c < L.ncolors; ++c)
+    int mx[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long sm[3] = {0, 0, 0};
+    for(int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < size; i += (int64_t)gridDim.x * 256)
     {
-        for(int R = 1; R <= 2; ++R)
+        const int v = desc[off + i];
+        const int d = v & 255, tl = (v >> 8) & 255, e = (v >> 16) & 255;
+        const int w1[3] = {d, e - (d + 1), e - tl};
+        for(int k = 0; k < 3; ++k)
         {
-            int wl = 0, wu = 0, wt = 0;
-            const int64_t b = L.offsets[c], e = L.offsets[c] + L.sizes[c];
-            for(int64_t r0 = b; r0 < e; r0 += R)
+            mx[k] = max(mx[k], w1[k]);
+            sm[k] += (unsigned long long)max(w1[k], 0);
+        }
+        if((i & 1) == 0)
+        {
+            int maxd = d, maxe = e, mind1 = d + 1, mintl = tl;
+            if(i + 1 < size)
             {
-                int maxd = 0, maxe = 0, mind1 = W + 1, mintl = W + 1;
-                for(int64_t r = r0; r < r0 + R && r < e; ++r)
-                {
-                    const int v = h[(size_t)r];
-                    const int d = v & 255, tl = (v >> 8) & 255, en = (v >> 16) & 255;
-                    maxd = std::max(maxd, d);
-                    maxe = std::max(maxe, en);
-                    mind1 = std::min(mind1, d + 1);
-                    mintl = std::min(mintl, tl);
-                }
-                wl = std::max(wl, maxd);
-                wu = std::max(wu, maxe - mind1);
-                wt = std::max(wt, maxe - mintl);
-                if(R == 1)
-                {
-                    L.slots[WIN_LOWER][c] += maxd;
-                    L.slots[WIN_UPPER][c] += maxe - mind1;
-                    L.slots[WIN_TAIL][c] += maxe - mintl;
-                }
+                const int u = desc[off + i + 1];
+                const int d2 = u & 255, tl2 = (u >> 8) & 255, e2 = (u >> 16) & 255;
+                maxd = max(maxd, d2);
+                maxe = max(maxe, e2);
+                mind1 = min(mind1, d2 + 1);
+                mintl = min(mintl, tl2);
             }
-            L.win[WIN_LOWER][R - 1][c] = (unsigned char)wl;
-            L.win[WIN_UPPER][R - 1][c] = (unsigned char)wu;
-            L.win[WIN_TAIL][R - 1][c] = (unsigned char)wt;
+            mx[3] = max(mx[3], maxd);
+            mx[4] = max(mx[4], maxe - mind1);
+            mx[5] = max(mx[5], maxe - mintl);
         }
     }
+    __shared__ int lmx[6][4];
+    __shared__ unsigned long long lsm[3][4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for(int k = 0; k < 6; ++k)
+    {
+        int v = mx[k];
+        for(int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+        if(lane == 0) lmx[k][wid] = v;
+    }
+    for(int k = 0; k < 3; ++k)
+    {
+        unsigned long long v = sm[k];
+        for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if(lane == 0) lsm[k][wid] = v;
+    }
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        for(int k = 0; k < 6; ++k) atomicMax(stats + k, max(max(lmx[k][0], lmx[k][1]), max(lmx[k][2], lmx[k][3])));
+        for(int k = 0; k < 3; ++k) atomicAdd(sums + k, lsm[k][0] + lsm[k][1] + lsm[k][2] + lsm[k][3]);
+    }
+}
+
+// Per colour and rows-per-thread, the widest slot window a sweep thread needs
+// (Level::win) and the window sums (Level::slots), from the row descriptors.
+static int window_stats(Level& L, hipStream_t s)
+{
+    const int nc = L.ncolors;
+    int* stats = nullptr;
+    unsigned long long* sums = nullptr;
+    HPCGX_HIP(hipMalloc(&stats, sizeof(int) * 6 * nc));
+    HPCGX_HIP(hipMalloc(&sums, sizeof(unsigned long long) * 3 * nc));
+    HPCGX_HIP(hipMemsetAsync(stats, 0, sizeof(int) * 6 * nc, s));
+    HPCGX_HIP(hipMemsetAsync(sums, 0, sizeof(unsigned long long) * 3 * nc, s));
+    for(int c = 0; c < nc; ++c)
+    {
+        if(L.sizes[c] == 0) continue;
+        const int64_t g = std::min<int64_t>(1024, ceil_div(L.sizes[c], 256));
+        hipLaunchKernelGGL(k_window_stats, dim3((unsigned)g), dim3(256), 0, s, (int64_t)L.offsets[c],
+                           (int64_t)L.sizes[c], L.desc, stats + 6 * c, sums + 3 * c);
+        HPCGX_HIP(hipGetLastError());
+    }
+    std::vector<int> hs((size_t)6 * nc);
+    std::vector<unsigned long long> hu((size_t)3 * nc);
+    HPCGX_HIP(hipMemcpyAsync(hs.data(), stats, sizeof(int) * 6 * nc, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipMemcpyAsync(hu.data(), sums, sizeof(unsigned long long) * 3 * nc, hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+    HPCGX_HIP(hipFree(stats));
+    HPCGX_HIP(hipFree(sums));
+    for(int c = 0; c < nc; ++c)
+        for(int k = 0; k < WIN_COUNT; ++k)
+        {
+            L.win[k][0][c] = (unsigned char)hs[6 * c + k];
+            L.win[k][1][c] = (unsigned char)hs[6 * c + 3 + k];
+            L.slots[k][c] = (int64_t)hu[3 * c + k];
+        }
     return 0;
 }
 
