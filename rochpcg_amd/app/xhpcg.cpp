@@ -71,6 +71,21 @@ struct Yaml
     }
 };
 
+// The run log of init.cpp:243-249: hpcgYYYYMMDDTHHMMSS.txt in the working
+// directory, opened at start; main.cpp:445,506-507 write the failure count and
+// one "Call [k] Scaled Residual [r]" line per timed set into it.
+FILE* open_log()
+{
+    const time_t now_t = time(nullptr);
+    const struct tm* tm = localtime(&now_t);
+    char name[64];
+    snprintf(name, sizeof name, "hpcg%04d%02d%02dT%02d%02d%02d.txt", 1900 + tm->tm_year, tm->tm_mon + 1,
+             tm->tm_mday, tm->tm_hour, tm->tm_min, tm->tm_sec);
+    FILE* f = fopen(name, "w");
+    if(!f) fprintf(stderr, "xhpcg: cannot open the log file %s\n", name);
+    return f;
+}
+
 // ---------------------------------------------------------------------------
 struct Params
 {
@@ -344,6 +359,7 @@ int main(int argc, char** argv)
 
     // device, stream and scratch before the setup timer, as HPCG_Init does
     if(HPCGX_Init(prm.device)) return 1;
+    FILE* log = open_log();
 
     // ---- setup (main.cpp:183-213) --------------------------------------
     double t = now();
@@ -444,6 +460,7 @@ int main(int argc, char** argv)
     {
         global_failure = 1;
         printf("Failed to reduce the residual %d times.\n", tol_failures);
+        if(log) fprintf(log, "Failed to reduce the residual %d times.\n", tol_failures);
     }
     printf("Optimized CG iterations per set: %d\n", opt_niters);
 
@@ -455,7 +472,13 @@ int main(int argc, char** argv)
     while((double)prm.rt - times[0] > 0.0 || sets < nsets)
     {
         HIPZeroVector(x);
-        if(CG(A, data, b, x, opt_niters, 0.0, niters, normr, normr0, times.data(), true, false)) global_failure = 1;
+        const int ierr = CG(A, data, b, x, opt_niters, 0.0, niters, normr, normr0, times.data(), true, false);
+        if(ierr)
+        {
+            global_failure = 1;
+            if(log) fprintf(log, "Error in call to CG: %d.\n\n", ierr);
+        }
+        if(log) fprintf(log, "Call [%d] Scaled Residual [%g]\n", sets, normr / normr0);
         scaled.push_back(normr / normr0);
         ++sets;
         const Model M = model(A, levels, sets, opt_niters);
@@ -638,5 +661,6 @@ int main(int argc, char** argv)
     HIPDeleteVector(b);
     HIPDeleteVector(x);
     HIPDeleteVector(xexact);   // DeleteMatrix(A) released geom with the hierarchy
+    if(log) fclose(log);
     return valid ? 0 : 2;
 }

@@ -116,3 +116,9 @@ def test_benchmark_driver_report(tmp_path, extra):
         assert opt == 50
     for k in ("DDOT", "WAXPBY", "SpMV", "MG", "Total"):
         assert float(y[f"Benchmark Time Summary/{k}"]) > 0.0
+    # the run log (init.cpp:243-249): one "Call [k] Scaled Residual [r]" line per timed set
+    logs = sorted(tmp_path.glob("hpcg*T*.txt"))
+    assert len(logs) == 1, logs
+    calls = [ln for ln in logs[0].read_text().splitlines() if ln.startswith("Call [")]
+    assert len(calls) == int(y["Iteration Count Information/Total number of optimized iterations"]) // opt
+    assert calls[0].startswith("Call [0] Scaled Residual [")
