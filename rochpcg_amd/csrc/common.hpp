@@ -104,7 +104,8 @@ struct Level
     int32_t* diag_idx = nullptr;  // [m] slot of the diagonal
     int32_t* desc = nullptr;      // [m] row descriptor: diagonal slot, first last-colour slot, valid slots
     double* inv_diag = nullptr;   // [m] 1/a_ii (f64)
-    float* diag_f = nullptr;      // [m] a_ii in f32 (mixed levels)
+    double* diag64 = nullptr;     // [m] a_ii (f64 levels)
+    __half* diag_h = nullptr;     // [m] a_ii (mixed levels)
     int32_t* perm = nullptr;      // [m] natural row -> multicolour row
     int32_t* colors = nullptr;    // [m] colour of each natural row
     int32_t* iperm_tmp = nullptr; // [m] multicolour row -> natural row (setup only)

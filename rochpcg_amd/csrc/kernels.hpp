@@ -169,6 +169,20 @@ __device__ __forceinline__ void ld(const T* p, T (&o)[R])
     }
 }
 
+template <int R>
+__device__ __forceinline__ void ld(const __half* p, __half (&o)[R])
+{
+    if constexpr(R == 1)
+        o[0] = *p;
+    else
+    {
+        typedef typename Ext<uint16_t, R>::type V;
+        const V w = *reinterpret_cast<const V*>(p);
+#pragma unroll
+        for(int k = 0; k < R; ++k) o[k] = __ushort_as_half(w[k]);
+    }
+}
+
 template <int R, typename T>
 __device__ __forceinline__ void st(T* p, const T (&v)[R])
 {
@@ -350,6 +364,43 @@ __device__ __forceinline__ void row_fwd_zero(int64_t m, int64_t row, const int32
     XT xv[NS][R];
     gather<NS, XT, R>(cc, x, xv);
     sub_window<NS, VT, XT, R>(cc, av, xv, s);
+#pragma unroll
+    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
+    st<R>(x + row, s);
+}
+
+// The same sweep with the loads led by the window rather than by the row
+// descriptor: the window [0, NS) starts at slot 0 whatever the row, so its
+// column and value loads, the diagonal (from its own array) and r are issued
+// together with the descriptor load; the descriptor only masks slots >= d
+// afterwards.  Slots in [d, NS) are read for rows with fewer lower entries
+// than the colour's widest (boundary rows), and their x is gathered but unused.
+template <int NS, typename VT, typename XT, typename RT, int R>
+__device__ __forceinline__ void row_fwd_zero_lead(int64_t m, int64_t row, const int32_t* __restrict__ col,
+                                                  const VT* __restrict__ val, const VT* __restrict__ dg,
+                                                  const int32_t* __restrict__ desc, const RT* __restrict__ r,
+                                                  XT* __restrict__ x)
+{
+    int32_t ds[R];
+    ld<R>(desc + row, ds);
+    int32_t cc[NS][R];
+    VT av[NS][R];
+#pragma unroll
+    for(int q = 0; q < NS; ++q)
+    {
+        ld_nt<R>(col + (int64_t)q * m + row, cc[q]);
+        ld_nt<R>(val + (int64_t)q * m + row, av[q]);
+    }
+    XT d[R], s[R];
+    ld_cvt<R>(dg + row, d);
+    ld_cvt<R>(r + row, s);
+    XT xv[NS][R];
+    gather<NS, XT, R>(cc, x, xv);
+#pragma unroll
+    for(int q = 0; q < NS; ++q)
+#pragma unroll
+        for(int k = 0; k < R; ++k)
+            if(q < desc_diag(ds[k]) && cc[q][k] >= 0) s[k] = fmaT(-cvt<XT>(av[q][k]), xv[q][k], s[k]);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);

@@ -75,11 +75,12 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
 
 template <int NS, typename VT, typename XT, typename RT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
-                                                 const VT* __restrict__ val, const int32_t* __restrict__ desc,
-                                                 const RT* __restrict__ r, XT* __restrict__ x, const int* gate)
+                                                 const VT* __restrict__ val, const VT* __restrict__ dg,
+                                                 const int32_t* __restrict__ desc, const RT* __restrict__ r,
+                                                 XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    row_fwd_zero<NS, VT, XT, RT, R>(m, row, col, val, desc, r, x);
+    row_fwd_zero_lead<NS, VT, XT, RT, R>(m, row, col, val, dg, desc, r, x);
 }
 
 template <int NS, typename VT, typename XT, bool STORE_U, int R, int BS>
@@ -303,12 +304,13 @@ __global__ void k_get_diag(int64_t m, const int32_t* __restrict__ dgi, const VT*
 }
 
 __global__ void k_set_diag(int64_t m, const int32_t* __restrict__ dgi, double* __restrict__ val,
-                           double* __restrict__ inv_diag, const double* __restrict__ d)
+                           double* __restrict__ dg, double* __restrict__ inv_diag, const double* __restrict__ d)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(r < m)
     {
         val[(int64_t)dgi[r] * m + r] = d[r];
+        dg[r] = d[r];
         if(inv_diag) inv_diag[r] = 1.0 / d[r];
     }
 }
@@ -400,6 +402,13 @@ const double* vals_of<double>(const Level& L) { return L.ell_val; }
 template <>
 const __half* vals_of<__half>(const Level& L) { return L.ell_h; }
 
+template <typename VT>
+static const VT* diag_of(const Level& L);
+template <>
+const double* diag_of<double>(const Level& L) { return L.diag64; }
+template <>
+const __half* diag_of<__half>(const Level& L) { return L.diag_h; }
+
 template <typename VT, typename XT, typename YT>
 static int spmv_t(const Level& L, const void* x, void* y, const RedArgs* ra, hipStream_t s, const int* gate)
 {
@@ -487,7 +496,7 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
             with_window(L.win[WIN_LOWER][R - 1][c], [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
                 hipLaunchKernelGGL((k_fwd_zero<NS, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, r, x, gate);
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, diag_of<VT>(L), L.desc, r, x, gate);
             });
         });
     case 1:
@@ -733,7 +742,7 @@ int launch_set_diag(Level& L, const double* d, hipStream_t s)
         set_error("diagonal replacement needs an f64 level");
         return -2;
     }
-    hipLaunchKernelGGL(k_set_diag, dim3(nblk(L.m, 256)), dim3(256), 0, s, L.m, L.diag_idx, L.ell_val, L.inv_diag, d);
+    hipLaunchKernelGGL(k_set_diag, dim3(nblk(L.m, 256)), dim3(256), 0, s, L.m, L.diag_idx, L.ell_val, L.diag64, L.inv_diag, d);
     HPCGX_KCHECK();
     return 0;
 }

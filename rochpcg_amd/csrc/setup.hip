@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_build_ell(Grid g, int64_t m, const int3
                                                    int32_t* __restrict__ ell_col, VT* __restrict__ ell_val,
                                                    int32_t* __restrict__ diag_idx, int32_t* __restrict__ desc,
                                                    int32_t last_off, double* __restrict__ inv_diag,
-                                                   float* __restrict__ diag_f)
+                                                   VT* __restrict__ dvals)
 {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if(r >= m) return;
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_build_ell(Grid g, int64_t m, const int3
     diag_idx[r] = dg;
     desc[r] = dg | (tl << 8) | (ne << 16);
     if(inv_diag) inv_diag[r] = 1.0 / 26.0;
-    if(diag_f) diag_f[r] = 26.0f;
+    dvals[r] = (VT)26.0f;
 }
 
 // Injection maps for coarse natural row k: f2c_nat = 2*kc per dimension
@@ -566,18 +566,19 @@ int build_level(Level& L, int nx, int ny, int nz, bool with_f64, bool with_f16, 
     {
         HPCGX_HIP(hipMalloc(&L.ell_val, sizeof(double) * W * m));
         HPCGX_HIP(hipMalloc(&L.inv_diag, sizeof(double) * m));
+        HPCGX_HIP(hipMalloc(&L.diag64, sizeof(double) * m));
         hipLaunchKernelGGL(k_build_ell<double>, dim3(grid_of(m, 256)), dim3(256), 0, s, g, m, L.perm,
                            L.iperm_tmp, L.ell_col, L.ell_val, L.diag_idx, L.desc, last_off, L.inv_diag,
-                           (float*)nullptr);
+                           L.diag64);
         HPCGX_HIP(hipGetLastError());
     }
     if(with_f16)
     {
         HPCGX_HIP(hipMalloc(&L.ell_h, sizeof(__half) * W * m));
-        HPCGX_HIP(hipMalloc(&L.diag_f, sizeof(float) * m));
+        HPCGX_HIP(hipMalloc(&L.diag_h, sizeof(__half) * m));
         hipLaunchKernelGGL(k_build_ell<__half>, dim3(grid_of(m, 256)), dim3(256), 0, s, g, m, L.perm,
                            L.iperm_tmp, L.ell_col, L.ell_h, L.diag_idx, L.desc, last_off, (double*)nullptr,
-                           L.diag_f);
+                           L.diag_h);
         HPCGX_HIP(hipGetLastError());
     }
     HPCGX_HIP(hipStreamSynchronize(s));
@@ -627,7 +628,8 @@ void free_level(Level& L)
     f(L.diag_idx);
     f(L.desc);
     f(L.inv_diag);
-    f(L.diag_f);
+    f(L.diag64);
+    f(L.diag_h);
     f(L.perm);
     f(L.colors);
     f(L.iperm_tmp);
