@@ -63,13 +63,18 @@ int hpcgx_level_info(const hpcgx_problem* p, int level, int64_t* info);
 int hpcgx_color_table(const hpcgx_problem* p, int level, int* sizes, int* offsets, int* round_sizes);
 /* device pointers of a level's arrays (SparseMatrix::ell_col_ind, ell_val,
  * diag_idx, inv_diag, perm; MGData::d_f2cOperator); any output may be NULL.
- * ptrs must hold 11 entries: ptrs[0..10] = ell_col_ind, ell_val (f64 or
+ * The ELL is column-major (entry p of row r at p*rows + r) with the entries of
+ * each row in the reference's ascending column order; the padding (-1, 0.0)
+ * sits between the diagonal and the upper entries, which end at slot 26,
+ * instead of at the row's tail (SparseMatrix.cpp:271-407 pads the tail).
+ * ptrs must hold 13 entries: ptrs[0..12] = ell_col_ind, ell_val (f64 or
  * NULL), ell_val_f16 (or NULL), diag_idx, inv_diag, perm (natural ->
  * multicolour), colors (natural), f2c (coarse natural -> fine natural),
  * f2c_fine (coarse natural -> fine multicolour row), f2c_coarse (coarse
  * natural -> coarse multicolour row), row descriptors (int32: diagonal slot
- * in bits 0-7, first slot of a last-colour column in bits 8-15, valid slots
- * in bits 16-23; no reference counterpart) */
+ * in bits 0-7, first slot of an off-diagonal last-colour column in bits 8-15,
+ * first upper slot in bits 16-23, 27 when absent; no reference counterpart), diagonal
+ * values f64 (or NULL) and f16 (or NULL) (no reference counterpart) */
 int hpcgx_level_arrays(const hpcgx_problem* p, int level, void** ptrs);
 /* device pointers of the CG data and system: ptrs[0..7] = b, x, xexact, r,
  * z (f64, or NULL), p, Ap, z (f32 mixed-mode, or NULL)  (CGData.hpp:55-61) */

@@ -233,7 +233,7 @@ class Problem:
     def level_arrays(self, level=0):
         """Device arrays of a level copied to host numpy arrays."""
         L = self.levels[level]
-        ptrs = (C.c_void_p * 11)()
+        ptrs = (C.c_void_p * 13)()
         _check(lib.hpcgx_level_arrays(self.h, level, ptrs))
         m, mc = L.m, L.mc
 
@@ -247,7 +247,8 @@ class Problem:
             inv_diag=get(ptrs[4], m, np.float64), perm=get(ptrs[5], m, np.int32),
             colors=get(ptrs[6], m, np.int32), f2c=get(ptrs[7], mc, np.int32),
             f2c_fine=get(ptrs[8], mc, np.int32), f2c_coarse=get(ptrs[9], mc, np.int32),
-            desc=get(ptrs[10], m, np.int32))
+            desc=get(ptrs[10], m, np.int32), diag=get(ptrs[11], m, np.float64),
+            diag_f16=get(ptrs[12], m, np.float16))
         return out
 
     def make_rhs(self, kind=0, seed=0):

@@ -68,16 +68,16 @@ __global__ __launch_bounds__(BS) void kb_spmv(const BSeg* __restrict__ segs, int
     if(DOT) block_partial<BS>(acc, S.ra.partial, lb);
 }
 
-// The sweeps take the slot-window bounds of the launch (the widest window of
-// the colour over all its problems, Level::win), like the single-problem
-// kernels.
+// The sweeps are compiled for the widest window of the launch's segments
+// (BLaunch::win), like the single-problem kernels.
 template <int NS, typename VT, typename XT, typename RT, int BS>
 __global__ __launch_bounds__(BS) void kb_fwd_zero(const BSeg* __restrict__ segs, int nseg)
 {
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    row_fwd_zero<NS, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x);
+    row_fwd_zero<NS, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+                                    (const RT*)S.r, (XT*)S.x);
 }
 
 template <int NS, typename VT, typename XT, int BS>
@@ -87,9 +87,11 @@ __global__ __launch_bounds__(BS) void kb_bwd_zero(const BSeg* __restrict__ segs,
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
     if(S.t)
-        row_bwd_zero<NS, VT, XT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, (XT*)S.t);
+        row_bwd_zero<NS, VT, XT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+                                          (XT*)S.x, (XT*)S.t);
     else
-        row_bwd_zero<NS, VT, XT, false, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (XT*)S.x, nullptr);
+        row_bwd_zero<NS, VT, XT, false, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+                                           (XT*)S.x, nullptr);
 }
 
 // every segment reuses its pre-smoother's upper sums (BLaunch::all_u)
@@ -99,8 +101,8 @@ __global__ __launch_bounds__(BS) void kb_fwd_full_u(const BSeg* __restrict__ seg
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    row_fwd_full_u<NA, NB, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x,
-                                          (XT*)S.t);
+    row_fwd_full_u<NA, NB, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg,
+                                          S.desc, (const RT*)S.r, (XT*)S.x, (XT*)S.t);
 }
 
 template <typename VT, typename XT, typename RT, int BS>
@@ -110,8 +112,8 @@ __global__ __launch_bounds__(BS) void kb_fwd_full(const BSeg* __restrict__ segs,
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
     if(S.use_u)
-        row_fwd_full_u<W, W, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const RT*)S.r, (XT*)S.x,
-                                            (XT*)S.t);
+        row_fwd_full_u<W, W, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg,
+                                            S.desc, (const RT*)S.r, (XT*)S.x, (XT*)S.t);
     else if(S.t)
         row_fwd_full<VT, XT, RT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
                                           (XT*)S.t);
@@ -126,7 +128,8 @@ __global__ __launch_bounds__(BS) void kb_bwd_full(const BSeg* __restrict__ segs,
     BSEG();
     const int64_t i = lb * BS + threadIdx.x;
     if(i >= S.size) return;
-    row_bwd_full<NS, VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, S.desc, (const XT*)S.t, (XT*)S.x);
+    row_bwd_full<NS, VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+                                (const XT*)S.t, (XT*)S.x);
 }
 
 template <typename VT, typename XT, typename RT, int BS>
@@ -369,7 +372,7 @@ struct Planner
     {
         int w = 0;
         for(Problem* P : B.probs)
-            if(c < P->L[l].ncolors) w = std::max<int>(w, P->L[l].win[wk][0][c]);
+            if(c < P->L[l].ncolors) w = std::max<int>(w, P->L[l].win[wk][c]);
         return w;
     }
 
@@ -446,6 +449,7 @@ struct Planner
     }
 
     const void* vals(const Level& L) { return prec == PREC_F64 ? (const void*)L.ell_val : (const void*)L.ell_h; }
+    const void* diags(const Level& L) { return prec == PREC_F64 ? (const void*)L.diag64 : (const void*)L.diag_h; }
     LevelVecs* vecs(Problem* P) { return prec == PREC_F64 ? P->v64 : P->v32; }
     void* z(Problem* P) { return prec == PREC_F64 ? (void*)P->z64 : (void*)P->z32; }
 
@@ -472,6 +476,7 @@ struct Planner
             s.col = L.ell_col;
             s.val = vals(L);
             s.desc = L.desc;
+            s.dg = diags(L);
             s.r = (l == 0) ? (const void*)P->r : vecs(P)[l].r;
             s.x = (l == 0) ? z(P) : vecs(P)[l].x;
             return true;

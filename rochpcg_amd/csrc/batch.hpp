@@ -20,6 +20,7 @@ struct BSeg
     const int32_t* col;
     const void* val;
     const int32_t* desc;   // row descriptors (Level::desc)
+    const void* dg;    // diagonal values (Level::diag64 / diag_h)
     const void* r;     // rhs / first vector operand
     void* x;           // solution / in-out operand
     void* t;           // SymGS partial sums (null: do not keep them); upper sums in a
@@ -59,7 +60,7 @@ struct BLaunch
     int dot;          // SpMV: fuse p'Ap
     int bf32;         // vector kernels: the second operand (z) is f32
     int block;        // threads per block
-    int win;          // sweeps: widest slot window of the launch's rows (Level::win, one row per thread)
+    int win;          // sweeps: widest slot window of the launch's segments (Level::win)
     int win_tail;     // post-smoother forward sweep: widest last-colour window
     int all_u;        // post-smoother forward sweep: every segment reuses its upper sums
     int nseg;

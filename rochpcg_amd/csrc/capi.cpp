@@ -128,8 +128,8 @@ int hpcgx_level_arrays(const hpcgx_problem* p, int level, void** ptrs)
     CHECK_LEVEL(p, level);
     const Level& L = p->P->L[level];
     void* v[] = {L.ell_col, L.ell_val, L.ell_h, L.diag_idx, L.inv_diag, L.perm, L.colors, L.f2c_nat, L.f2c_fine,
-                 L.f2c_coarse, L.desc};
-    for(int i = 0; i < 11; ++i) ptrs[i] = v[i];
+                 L.f2c_coarse, L.desc, L.diag64, L.diag_h};
+    for(int i = 0; i < 13; ++i) ptrs[i] = v[i];
     return 0;
 }
 

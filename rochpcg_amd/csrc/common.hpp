@@ -57,13 +57,14 @@ struct Shape
     int rows = 1;      // consecutive rows per thread: 1 or 2
 };
 
-// Slot windows of the sweeps (see load_window): per colour and rows-per-thread
-// R in {1, 2}, the largest window a thread needs
+// Slot windows of the sweeps (see load_window): per colour, the widest window
+// of its rows in the split-padded ELL (the same for every rows-per-thread R,
+// since each window starts at a fixed slot)
 enum WinKind : int
 {
-    WIN_LOWER = 0,   // [0, d): max over threads of max d
-    WIN_UPPER = 1,   // (d, e): max over threads of max e - min (d+1)
-    WIN_TAIL = 2,    // [tl, e): max over threads of max e - min tl
+    WIN_LOWER = 0,   // [0, d): max d
+    WIN_UPPER = 1,   // [us, W): max W - us (upper entries)
+    WIN_TAIL = 2,    // [tl, W): max W - tl (last-colour entries)
     WIN_COUNT = 3
 };
 
@@ -119,8 +120,7 @@ struct Level
     // only last-colour rows, so the post-smoother may reuse the pre-smoother's
     // upper sums over the other colours
     bool u_ok = false;
-    unsigned char win[WIN_COUNT][2][MAX_COLORS] = {};   // [kind][R-1][colour]
-    int64_t slots[WIN_COUNT][MAX_COLORS] = {};         // [kind][colour]: window slots summed over rows
+    unsigned char win[WIN_COUNT][MAX_COLORS] = {};   // [kind][colour]: slots a sweep thread loads per row
 
     Shape shp[2][SH_COUNT];   // [precision][operator]
 };

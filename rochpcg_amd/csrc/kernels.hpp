@@ -261,48 +261,53 @@ __device__ __forceinline__ void row_spmv(int64_t m, int64_t row, const int32_t* 
             if(cc[p][k] >= 0) s[k] = fmaT(cvt<XT>(av[p][k]), xv[p][k], s[k]);
 }
 
-// Row descriptor, one int32 per row built with the ELL (slots hold the
-// columns in ascending multicolour order, padding last): bits 0-7 the slot of
-// the diagonal d, bits 8-15 the first slot whose column lies in the last
-// colour tl (the end when there is none), bits 16-23 the number of valid
-// slots e.  So [0, d) are the lower entries, (d, tl) the upper entries of the
-// colours before the last and [tl, e) the entries of the last colour.
+// ELL layout of the sweeps ("split" padding).  The entries of a row keep
+// the ascending multicolour column order of the reference's ELL, but the
+// padding sits between the diagonal and the upper entries instead of at the
+// tail: slots [0, d) hold the lower entries, slot d the diagonal, slots
+// [us, W) the upper entries, right-aligned, and the last-colour entries (the
+// largest columns) are their tail [tl, W).  Every window a sweep reads then
+// starts at a slot that does not depend on the row -- 0 for the lower
+// entries, W - n for the upper or last-colour ones -- so its loads are issued
+// with the row's descriptor load instead of after it.
+//
+// Row descriptor, one int32 per row: bits 0-7 the diagonal slot d, bits 8-15
+// the first slot of an off-diagonal last-colour column tl, bits 16-23 the
+// first upper slot us (W when there is none).
 __device__ __forceinline__ int desc_diag(int32_t v) { return v & 255; }
 __device__ __forceinline__ int desc_tail(int32_t v) { return (v >> 8) & 255; }
-__device__ __forceinline__ int desc_end(int32_t v) { return (v >> 16) & 255; }
+__device__ __forceinline__ int desc_upper(int32_t v) { return (v >> 16) & 255; }
 
-// Slot windows.  A sweep touches one or two contiguous slot ranges of each
-// row ([0, d), (d, e) or [tl, e)).  A thread's window starts at slot `start`
-// (the smallest start over its R rows) and spans NS slots: NS is a
-// compile-time bound the host picks per colour from the rows' descriptors
-// (Level::win), so light colours keep few registers live and run at higher
-// occupancy.  Slot start+q of row k is used iff lo[k] <= start+q < hi[k];
-// unused slots carry column -1.
+// Slot windows.  A sweep reads NS consecutive slots from a fixed start: 0
+// for the lower entries, W - NS for the upper or last-colour ones.  NS is a
+// compile-time bound the host picks per colour (Level::win, rounded up to a
+// bucket; the buckets are the windows the parity colourings produce, so no
+// slot is read in vain there), so light colours keep few registers live.
+// The loads are unconditional -- no branch between them, so they are all in
+// flight at once -- and which loaded slots a row uses is decided afterwards
+// from its descriptor.  NS may be 0 (a colour with no lower entries); arrays
+// then keep one unused element.
+constexpr int slot_dim(int n) { return n > 0 ? n : 1; }
+
 template <int NS, typename VT, int R>
 __device__ __forceinline__ void load_window(int64_t m, int64_t row, const int32_t* __restrict__ col,
-                                            const VT* __restrict__ val, int start, const int (&lo)[R],
-                                            const int (&hi)[R], int32_t (&cc)[NS][R], VT (&av)[NS][R])
+                                            const VT* __restrict__ val, int start,
+                                            int32_t (&cc)[slot_dim(NS)][R], VT (&av)[slot_dim(NS)][R])
 {
 #pragma unroll
     for(int q = 0; q < NS; ++q)
     {
-        const int p = start + q;
-        bool any = false;
-#pragma unroll
-        for(int k = 0; k < R; ++k) any = any || (p >= lo[k] && p < hi[k]);
-        if(any)
-        {
-            ld_nt<R>(col + (int64_t)p * m + row, cc[q]);
-            ld_nt<R>(val + (int64_t)p * m + row, av[q]);
-        }
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(!(p >= lo[k] && p < hi[k])) cc[q][k] = -1;
+        ld_nt<R>(col + (int64_t)(start + q) * m + row, cc[q]);
+        ld_nt<R>(val + (int64_t)(start + q) * m + row, av[q]);
     }
 }
 
+// Neighbour values of every loaded slot with a column (a slot the row does
+// not use reads a value that is then ignored: its own diagonal, or a
+// neighbour outside the window's kind -- never a row this launch writes).
 template <int NS, typename XT, int R>
-__device__ __forceinline__ void gather(const int32_t (&cc)[NS][R], const XT* __restrict__ x, XT (&xv)[NS][R])
+__device__ __forceinline__ void gather(const int32_t (&cc)[slot_dim(NS)][R], const XT* __restrict__ x,
+                                       XT (&xv)[slot_dim(NS)][R])
 {
 #pragma unroll
     for(int q = 0; q < NS; ++q)
@@ -310,44 +315,40 @@ __device__ __forceinline__ void gather(const int32_t (&cc)[NS][R], const XT* __r
         for(int k = 0; k < R; ++k) xv[q][k] = (cc[q][k] >= 0) ? x[cc[q][k]] : (XT)0;
 }
 
-// s[k] -= sum over the used slots of row k, ascending
+// s -= a x over the window's slots in [lo, hi) of each row, ascending.
 template <int NS, typename VT, typename XT, int R>
-__device__ __forceinline__ void sub_window(const int32_t (&cc)[NS][R], const VT (&av)[NS][R],
-                                           const XT (&xv)[NS][R], XT (&s)[R])
+__device__ __forceinline__ void sub_window(int start, const int (&lo)[R], const int (&hi)[R],
+                                           const int32_t (&cc)[slot_dim(NS)][R],
+                                           const VT (&av)[slot_dim(NS)][R], const XT (&xv)[slot_dim(NS)][R],
+                                           XT (&s)[R])
 {
 #pragma unroll
     for(int q = 0; q < NS; ++q)
 #pragma unroll
         for(int k = 0; k < R; ++k)
-            if(cc[q][k] >= 0) s[k] = fmaT(-cvt<XT>(av[q][k]), xv[q][k], s[k]);
-}
-
-template <typename VT, typename XT, int R>
-__device__ __forceinline__ void diag_vals(int64_t m, int64_t row, const VT* __restrict__ val, const int32_t (&ds)[R],
-                                          XT (&d)[R])
-{
-#pragma unroll
-    for(int k = 0; k < R; ++k) d[k] = cvt<XT>(val[(int64_t)desc_diag(ds[k]) * m + row + k]);
-}
-
-template <int R>
-__device__ __forceinline__ int min_of(const int (&v)[R])
-{
-    int r = v[0];
-#pragma unroll
-    for(int k = 1; k < R; ++k) r = v[k] < r ? v[k] : r;
-    return r;
+            if(start + q >= lo[k] && start + q < hi[k] && cc[q][k] >= 0)
+                s[k] = fmaT(-cvt<XT>(av[q][k]), xv[q][k], s[k]);
 }
 
 // Zero-initial-guess forward sweep: only entries left of the diagonal
-// (earlier colours) are non-zero in x.  Window [0, d), NS >= max d.
+// (earlier colours) are non-zero in x.  Window [0, NS), NS >= the colour's
+// largest d; the diagonal comes from its own array (dg).
 template <int NS, typename VT, typename XT, typename RT, int R>
 __device__ __forceinline__ void row_fwd_zero(int64_t m, int64_t row, const int32_t* __restrict__ col,
-                                             const VT* __restrict__ val, const int32_t* __restrict__ desc,
-                                             const RT* __restrict__ r, XT* __restrict__ x)
+                                             const VT* __restrict__ val, const VT* __restrict__ dg,
+                                             const int32_t* __restrict__ desc, const RT* __restrict__ r,
+                                             XT* __restrict__ x)
 {
     int32_t ds[R];
     ld<R>(desc + row, ds);
+    int32_t cc[slot_dim(NS)][R];
+    VT av[slot_dim(NS)][R];
+    load_window<NS, VT, R>(m, row, col, val, 0, cc, av);
+    XT d[R], s[R];
+    ld_cvt<R>(dg + row, d);
+    ld_cvt<R>(r + row, s);
+    XT xv[slot_dim(NS)][R];
+    gather<NS, XT, R>(cc, x, xv);
     int lo[R], hi[R];
 #pragma unroll
     for(int k = 0; k < R; ++k)
@@ -355,88 +356,45 @@ __device__ __forceinline__ void row_fwd_zero(int64_t m, int64_t row, const int32
         lo[k] = 0;
         hi[k] = desc_diag(ds[k]);
     }
-    int32_t cc[NS][R];
-    VT av[NS][R];
-    load_window<NS, VT, R>(m, row, col, val, 0, lo, hi, cc, av);
-    XT d[R], s[R];
-    diag_vals<VT, XT, R>(m, row, val, ds, d);
-    ld_cvt<R>(r + row, s);
-    XT xv[NS][R];
-    gather<NS, XT, R>(cc, x, xv);
-    sub_window<NS, VT, XT, R>(cc, av, xv, s);
+    sub_window<NS, VT, XT, R>(0, lo, hi, cc, av, xv, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);
 }
 
-// The same sweep with the loads led by the window rather than by the row
-// descriptor: the window [0, NS) starts at slot 0 whatever the row, so its
-// column and value loads, the diagonal (from its own array) and r are issued
-// together with the descriptor load; the descriptor only masks slots >= d
-// afterwards.  Slots in [d, NS) are read for rows with fewer lower entries
-// than the colour's widest (boundary rows), and their x is gathered but unused.
-template <int NS, typename VT, typename XT, typename RT, int R>
-__device__ __forceinline__ void row_fwd_zero_lead(int64_t m, int64_t row, const int32_t* __restrict__ col,
-                                                  const VT* __restrict__ val, const VT* __restrict__ dg,
-                                                  const int32_t* __restrict__ desc, const RT* __restrict__ r,
-                                                  XT* __restrict__ x)
-{
-    int32_t ds[R];
-    ld<R>(desc + row, ds);
-    int32_t cc[NS][R];
-    VT av[NS][R];
-#pragma unroll
-    for(int q = 0; q < NS; ++q)
-    {
-        ld_nt<R>(col + (int64_t)q * m + row, cc[q]);
-        ld_nt<R>(val + (int64_t)q * m + row, av[q]);
-    }
-    XT d[R], s[R];
-    ld_cvt<R>(dg + row, d);
-    ld_cvt<R>(r + row, s);
-    XT xv[NS][R];
-    gather<NS, XT, R>(cc, x, xv);
-#pragma unroll
-    for(int q = 0; q < NS; ++q)
-#pragma unroll
-        for(int k = 0; k < R; ++k)
-            if(q < desc_diag(ds[k]) && cc[q][k] >= 0) s[k] = fmaT(-cvt<XT>(av[q][k]), xv[q][k], s[k]);
-#pragma unroll
-    for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
-    st<R>(x + row, s);
-}
-
-// Zero-initial-guess backward sweep: x = (x*d - U x) / d over the window
-// (d, e), NS >= max e - min(d+1).  With STORE_U it also keeps u = the sum
-// over (d, tl) of a*x -- the upper entries outside the last colour, whose x
-// the coarse-grid correction leaves alone -- for the post-smoother's forward
-// sweep (row_fwd_full_u); u is read off the running sum (x*d minus its value
-// at slot tl), so it costs no second accumulation.
+// Zero-initial-guess backward sweep: x = (x*d - U x) / d over the upper
+// entries, window [W - NS, W), NS >= the colour's largest upper count.  With
+// STORE_U it also keeps u = the sum over [us, tl) of a*x -- the upper entries
+// outside the last colour, whose x the coarse-grid correction leaves alone --
+// for the post-smoother's forward sweep (row_fwd_full_u); u is read off the
+// running sum (x*d minus its value on reaching slot tl), so it costs no second
+// accumulation.
 template <int NS, typename VT, typename XT, bool STORE_U, int R>
 __device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32_t* __restrict__ col,
-                                             const VT* __restrict__ val, const int32_t* __restrict__ desc,
-                                             XT* __restrict__ x, XT* __restrict__ u)
+                                             const VT* __restrict__ val, const VT* __restrict__ dg,
+                                             const int32_t* __restrict__ desc, XT* __restrict__ x,
+                                             XT* __restrict__ u)
 {
     int32_t ds[R];
     ld<R>(desc + row, ds);
+    const int start = W - NS;
+    int32_t cc[slot_dim(NS)][R];
+    VT av[slot_dim(NS)][R];
+    load_window<NS, VT, R>(m, row, col, val, start, cc, av);
+    XT d[R], s[R];
+    ld_cvt<R>(dg + row, d);
+    ld<R>(x + row, s);
+#pragma unroll
+    for(int k = 0; k < R; ++k) s[k] = s[k] * d[k];
+    XT xv[slot_dim(NS)][R];
+    gather<NS, XT, R>(cc, x, xv);
     int lo[R], hi[R];
 #pragma unroll
     for(int k = 0; k < R; ++k)
     {
-        lo[k] = desc_diag(ds[k]) + 1;
-        hi[k] = desc_end(ds[k]);
+        lo[k] = desc_upper(ds[k]);
+        hi[k] = W;
     }
-    const int start = min_of<R>(lo);
-    int32_t cc[NS][R];
-    VT av[NS][R];
-    load_window<NS, VT, R>(m, row, col, val, start, lo, hi, cc, av);
-    XT d[R], s[R];
-    diag_vals<VT, XT, R>(m, row, val, ds, d);
-    ld<R>(x + row, s);
-#pragma unroll
-    for(int k = 0; k < R; ++k) s[k] = s[k] * d[k];
-    XT xv[NS][R];
-    gather<NS, XT, R>(cc, x, xv);
     if constexpr(STORE_U)
     {
         // u = x*d - (the running sum when it reaches the first last-colour
@@ -460,7 +418,7 @@ __device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32
                     snap[k] = s[k];
                     taken[k] = true;
                 }
-                if(cc[q][k] >= 0) s[k] = fmaT(-cvt<XT>(av[q][k]), xv[q][k], s[k]);
+                if(start + q >= lo[k] && cc[q][k] >= 0) s[k] = fmaT(-cvt<XT>(av[q][k]), xv[q][k], s[k]);
             }
         XT uu[R];
 #pragma unroll
@@ -469,7 +427,7 @@ __device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32
     }
     else
     {
-        sub_window<NS, VT, XT, R>(cc, av, xv, s);
+        sub_window<NS, VT, XT, R>(start, lo, hi, cc, av, xv, s);
     }
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
@@ -532,16 +490,29 @@ __device__ __forceinline__ void row_fwd_full(int64_t m, int64_t row, const int32
 // when the coarse-grid correction changed only rows of the last colour:
 // x = (r - L x - u - T x) / d with u = t[row] from row_bwd_zero<STORE_U> (the
 // upper entries outside the last colour at their unchanged x) and T the
-// last-colour entries [tl, e) at their corrected x.  Reads the lower, the
-// diagonal and the last-colour slots only (windows of NA >= max d and
-// NB >= max e - min tl slots); t[row] becomes r - L x.
+// last-colour entries [tl, W) at their corrected x.  Reads the lower, the
+// diagonal and the last-colour slots only: windows [0, NA) and [W - NB, W);
+// t[row] becomes r - L x.
 template <int NA, int NB, typename VT, typename XT, typename RT, int R>
 __device__ __forceinline__ void row_fwd_full_u(int64_t m, int64_t row, const int32_t* __restrict__ col,
-                                               const VT* __restrict__ val, const int32_t* __restrict__ desc,
+                                               const VT* __restrict__ val,
+                                               const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                const RT* __restrict__ r, XT* __restrict__ x, XT* __restrict__ t)
 {
     int32_t ds[R];
     ld<R>(desc + row, ds);
+    const int bstart = W - NB;
+    int32_t ca[slot_dim(NA)][R], cb[slot_dim(NB)][R];
+    VT aa[slot_dim(NA)][R], ab[slot_dim(NB)][R];
+    load_window<NA, VT, R>(m, row, col, val, 0, ca, aa);
+    load_window<NB, VT, R>(m, row, col, val, bstart, cb, ab);
+    XT d[R], s[R], uu[R];
+    ld_cvt<R>(dg + row, d);
+    ld_cvt<R>(r + row, s);
+    ld<R>(t + row, uu);
+    XT xa[slot_dim(NA)][R], xb[slot_dim(NB)][R];
+    gather<NA, XT, R>(ca, x, xa);
+    gather<NB, XT, R>(cb, x, xb);
     int alo[R], ahi[R], blo[R], bhi[R];
 #pragma unroll
     for(int k = 0; k < R; ++k)
@@ -549,21 +520,9 @@ __device__ __forceinline__ void row_fwd_full_u(int64_t m, int64_t row, const int
         alo[k] = 0;
         ahi[k] = desc_diag(ds[k]);
         blo[k] = desc_tail(ds[k]);
-        bhi[k] = desc_end(ds[k]);
+        bhi[k] = W;
     }
-    const int bstart = min_of<R>(blo);
-    int32_t ca[NA][R], cb[NB][R];
-    VT aa[NA][R], ab[NB][R];
-    load_window<NA, VT, R>(m, row, col, val, 0, alo, ahi, ca, aa);
-    load_window<NB, VT, R>(m, row, col, val, bstart, blo, bhi, cb, ab);
-    XT d[R], s[R], uu[R];
-    diag_vals<VT, XT, R>(m, row, val, ds, d);
-    ld_cvt<R>(r + row, s);
-    ld<R>(t + row, uu);
-    XT xa[NA][R], xb[NB][R];
-    gather<NA, XT, R>(ca, x, xa);
-    gather<NB, XT, R>(cb, x, xb);
-    sub_window<NA, VT, XT, R>(ca, aa, xa, s);
+    sub_window<NA, VT, XT, R>(0, alo, ahi, ca, aa, xa, s);
     XT tt[R];
 #pragma unroll
     for(int k = 0; k < R; ++k)
@@ -571,38 +530,40 @@ __device__ __forceinline__ void row_fwd_full_u(int64_t m, int64_t row, const int
         tt[k] = s[k];
         s[k] = s[k] - uu[k];
     }
-    sub_window<NB, VT, XT, R>(cb, ab, xb, s);
+    sub_window<NB, VT, XT, R>(bstart, blo, bhi, cb, ab, xb, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);
     st<R>(t + row, tt);
 }
 
-// Full backward sweep from t: x = (t - U x) / d over the window (d, e).
+// Full backward sweep from t: x = (t - U x) / d over the upper entries,
+// window [W - NS, W).
 template <int NS, typename VT, typename XT, int R>
 __device__ __forceinline__ void row_bwd_full(int64_t m, int64_t row, const int32_t* __restrict__ col,
-                                             const VT* __restrict__ val, const int32_t* __restrict__ desc,
-                                             const XT* __restrict__ t, XT* __restrict__ x)
+                                             const VT* __restrict__ val, const VT* __restrict__ dg,
+                                             const int32_t* __restrict__ desc, const XT* __restrict__ t,
+                                             XT* __restrict__ x)
 {
     int32_t ds[R];
     ld<R>(desc + row, ds);
+    const int start = W - NS;
+    int32_t cc[slot_dim(NS)][R];
+    VT av[slot_dim(NS)][R];
+    load_window<NS, VT, R>(m, row, col, val, start, cc, av);
+    XT d[R], s[R];
+    ld_cvt<R>(dg + row, d);
+    ld<R>(t + row, s);
+    XT xv[slot_dim(NS)][R];
+    gather<NS, XT, R>(cc, x, xv);
     int lo[R], hi[R];
 #pragma unroll
     for(int k = 0; k < R; ++k)
     {
-        lo[k] = desc_diag(ds[k]) + 1;
-        hi[k] = desc_end(ds[k]);
+        lo[k] = desc_upper(ds[k]);
+        hi[k] = W;
     }
-    const int start = min_of<R>(lo);
-    int32_t cc[NS][R];
-    VT av[NS][R];
-    load_window<NS, VT, R>(m, row, col, val, start, lo, hi, cc, av);
-    XT d[R], s[R];
-    diag_vals<VT, XT, R>(m, row, val, ds, d);
-    ld<R>(t + row, s);
-    XT xv[NS][R];
-    gather<NS, XT, R>(cc, x, xv);
-    sub_window<NS, VT, XT, R>(cc, av, xv, s);
+    sub_window<NS, VT, XT, R>(start, lo, hi, cc, av, xv, s);
 #pragma unroll
     for(int k = 0; k < R; ++k) s[k] = s[k] / d[k];
     st<R>(x + row, s);
@@ -637,33 +598,37 @@ __device__ __forceinline__ void row_restrict(int64_t m, int64_t k, const int32_t
 // host side: dispatch of the slot-window bounds (shared by ops.hip and
 // batch.hip)
 
-// Compile-time slot-window bounds.  The bucket edges sit at the window sizes
-// of the 27-point operator's parity colourings (lower: 0 2 10 12 16 22 26,
-// upper: 4 10 14 16 24 26 per colour), so each colour's kernel keeps only the
-// registers its rows need; any larger window takes the next bucket up.
+// Compile-time slot-window bounds.  The buckets are the window sizes of the
+// 27-point operator's parity colourings (lower: 0 2 10 12 16 22 26, upper:
+// 4 10 14 16 24 26 per colour), so those colours read no slot in vain and
+// keep only the registers they need; any other window takes the next bucket
+// up (at most 26: the diagonal always has a slot of its own).
 template <typename F>
 static void with_window(int w, F&& f)
 {
     using std::integral_constant;
-    if(w <= 4)
-        f(integral_constant<int, 4>{});
-    else if(w <= 10)
-        f(integral_constant<int, 10>{});
-    else if(w <= 12)
-        f(integral_constant<int, 12>{});
-    else if(w <= 16)
-        f(integral_constant<int, 16>{});
-    else if(w <= 22)
-        f(integral_constant<int, 22>{});
-    else if(w <= 24)
-        f(integral_constant<int, 24>{});
-    else
-        f(integral_constant<int, W>{});
+#define HPCGX_WIN(N)                          \
+    if(w <= (N))                              \
+    {                                         \
+        f(integral_constant<int, N>{});       \
+        return;                               \
+    }
+    HPCGX_WIN(0)
+    HPCGX_WIN(2)
+    HPCGX_WIN(4)
+    HPCGX_WIN(10)
+    HPCGX_WIN(12)
+    HPCGX_WIN(14)
+    HPCGX_WIN(16)
+    HPCGX_WIN(22)
+    HPCGX_WIN(24)
+#undef HPCGX_WIN
+    f(integral_constant<int, W - 1>{});
 }
 
 // (lower, last-colour) window pairs of the post-smoother's forward sweep, as
 // they occur per colour of the parity colourings; anything else takes the
-// smallest listed pair covering it, or (27, 27)
+// first listed pair covering it, or (26, 26)
 template <typename F>
 static void with_window_pair(int wl, int wt, F&& f)
 {
@@ -674,18 +639,16 @@ static void with_window_pair(int wl, int wt, F&& f)
         f(integral_constant<int, A>{}, integral_constant<int, B>{});      \
         return;                                                           \
     }
-    HPCGX_PAIR(4, 4)
-    HPCGX_PAIR(4, 8)
+    HPCGX_PAIR(0, 8)
+    HPCGX_PAIR(2, 4)
     HPCGX_PAIR(10, 2)
     HPCGX_PAIR(12, 2)
-    HPCGX_PAIR(12, 4)
     HPCGX_PAIR(16, 2)
     HPCGX_PAIR(16, 4)
     HPCGX_PAIR(22, 4)
-    HPCGX_PAIR(24, 4)
-    HPCGX_PAIR(W, 8)
+    HPCGX_PAIR(W - 1, 8)
 #undef HPCGX_PAIR
-    f(integral_constant<int, W>{}, integral_constant<int, W>{});
+    f(integral_constant<int, W - 1>{}, integral_constant<int, W - 1>{});
 }
 
 }  // namespace hpcgx

@@ -74,22 +74,21 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
     const int64_t row = off + i;
 
 template <int NS, typename VT, typename XT, typename RT, int R, int BS>
-__global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
-                                                 const VT* __restrict__ val, const VT* __restrict__ dg,
-                                                 const int32_t* __restrict__ desc, const RT* __restrict__ r,
-                                                 XT* __restrict__ x, const int* gate)
+__global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col, const VT* __restrict__ val,
+                                                 const VT* __restrict__ dg, const int32_t* __restrict__ desc,
+                                                 const RT* __restrict__ r, XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    row_fwd_zero_lead<NS, VT, XT, RT, R>(m, row, col, val, dg, desc, r, x);
+    row_fwd_zero<NS, VT, XT, RT, R>(m, row, col, val, dg, desc, r, x);
 }
 
 template <int NS, typename VT, typename XT, bool STORE_U, int R, int BS>
-__global__ __launch_bounds__(BS) void k_bwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
-                                                 const VT* __restrict__ val, const int32_t* __restrict__ desc,
+__global__ __launch_bounds__(BS) void k_bwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col, const VT* __restrict__ val,
+                                                 const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                  XT* __restrict__ x, XT* __restrict__ u, const int* gate)
 {
     SWEEP_ROW();
-    row_bwd_zero<NS, VT, XT, STORE_U, R>(m, row, col, val, desc, x, u);
+    row_bwd_zero<NS, VT, XT, STORE_U, R>(m, row, col, val, dg, desc, x, u);
 }
 
 template <typename VT, typename XT, typename RT, bool STORE_T, int R, int BS>
@@ -104,20 +103,21 @@ __global__ __launch_bounds__(BS) void k_fwd_full(int64_t m, int64_t off, int64_t
 template <int NA, int NB, typename VT, typename XT, typename RT, int R, int BS>
 __global__ __launch_bounds__(BS) void k_fwd_full_u(int64_t m, int64_t off, int64_t size,
                                                    const int32_t* __restrict__ col, const VT* __restrict__ val,
-                                                   const int32_t* __restrict__ desc, const RT* __restrict__ r,
-                                                   XT* __restrict__ x, XT* __restrict__ t, const int* gate)
+                                                   const VT* __restrict__ dg, const int32_t* __restrict__ desc,
+                                                   const RT* __restrict__ r, XT* __restrict__ x, XT* __restrict__ t,
+                                                   const int* gate)
 {
     SWEEP_ROW();
-    row_fwd_full_u<NA, NB, VT, XT, RT, R>(m, row, col, val, desc, r, x, t);
+    row_fwd_full_u<NA, NB, VT, XT, RT, R>(m, row, col, val, dg, desc, r, x, t);
 }
 
 template <int NS, typename VT, typename XT, int R, int BS>
-__global__ __launch_bounds__(BS) void k_bwd_full(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
-                                                 const VT* __restrict__ val, const int32_t* __restrict__ desc,
+__global__ __launch_bounds__(BS) void k_bwd_full(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col, const VT* __restrict__ val,
+                                                 const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                  const XT* __restrict__ t, XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    row_bwd_full<NS, VT, XT, R>(m, row, col, val, desc, t, x);
+    row_bwd_full<NS, VT, XT, R>(m, row, col, val, dg, desc, t, x);
 }
 #undef SWEEP_ROW
 
@@ -456,7 +456,7 @@ static int per_color(const Level& L, int prec, int which, int wk, int c_first, i
     for(int c = c_first; step > 0 ? c <= c_last : c >= c_last; c += step)
     {
         if(L.sizes[c] == 0) continue;
-        const bool wide = (wk < 0) || L.win[wk][0][c] > 16;
+        const bool wide = (wk < 0) || L.win[wk][c] > 16;
         const Shape sh = fit(L, L.shp[prec][wide ? which : which + SH_NARROW]);
         HPCGX_TRY(with_shape(sh, [&](auto RR, auto BB) -> int {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
@@ -485,6 +485,7 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
     XT* x = (XT*)xv;
     XT* t = (XT*)tv;
     const VT* val = vals_of<VT>(L);
+    const VT* dg = diag_of<VT>(L);
     const int last = L.ncolors - 1;
     const int pr = std::is_same<VT, double>::value ? PREC_F64 : PREC_MIXED;
     using std::integral_constant;
@@ -493,10 +494,11 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
     case 0:
         return per_color(L, pr, SH_FWD_ZERO, WIN_LOWER, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
-            with_window(L.win[WIN_LOWER][R - 1][c], [&](auto NN) {
+            const int n = L.win[WIN_LOWER][c];
+            with_window(n, [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
                 hipLaunchKernelGGL((k_fwd_zero<NS, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], sz, L.ell_col, val, diag_of<VT>(L), L.desc, r, x, gate);
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, dg, L.desc, r, x, gate);
             });
         });
     case 1:
@@ -504,14 +506,15 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
         if(last < 1) return 0;
         return per_color(L, pr, SH_BWD_ZERO, WIN_UPPER, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
-            with_window(L.win[WIN_UPPER][R - 1][c], [&](auto NN) {
+            const int n = L.win[WIN_UPPER][c];
+            with_window(n, [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
                 if(part == 4)
                     hipLaunchKernelGGL((k_bwd_zero<NS, VT, XT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                       (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, x, t, gate);
+                                       (int64_t)L.offsets[c], sz, L.ell_col, val, dg, L.desc, x, t, gate);
                 else
                     hipLaunchKernelGGL((k_bwd_zero<NS, VT, XT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                       (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, x, t, gate);
+                                       (int64_t)L.offsets[c], sz, L.ell_col, val, dg, L.desc, x, t, gate);
             });
         });
     case 2:
@@ -520,10 +523,12 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             if(c < last && part == 5)
             {
-                with_window_pair(L.win[WIN_LOWER][R - 1][c], L.win[WIN_TAIL][R - 1][c], [&](auto NA_, auto NB_) {
+                const int na = L.win[WIN_LOWER][c], nb = L.win[WIN_TAIL][c];
+                with_window_pair(na, nb, [&](auto NA_, auto NB_) {
                     constexpr int NA = decltype(NA_)::value, NB = decltype(NB_)::value;
                     hipLaunchKernelGGL((k_fwd_full_u<NA, NB, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                       (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, r, x, t, gate);
+                                       (int64_t)L.offsets[c], sz, L.ell_col, val, dg, L.desc, r, x, t,
+                                       gate);
                 });
             }
             else if(c < last)
@@ -537,15 +542,31 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
         if(last < 1) return 0;
         return per_color(L, pr, SH_BWD_FULL, WIN_UPPER, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
-            with_window(L.win[WIN_UPPER][R - 1][c], [&](auto NN) {
+            const int n = L.win[WIN_UPPER][c];
+            with_window(n, [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
                 hipLaunchKernelGGL((k_bwd_full<NS, VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], sz, L.ell_col, val, L.desc, (const XT*)t, x, gate);
+                                   (int64_t)L.offsets[c], sz, L.ell_col, val, dg, L.desc, (const XT*)t, x, gate);
             });
         });
     }
     set_error("invalid sweep part");
     return -2;
+}
+
+int window_slots(int w)
+{
+    int ns = 0;
+    with_window(w, [&](auto NN) { ns = decltype(NN)::value; });
+    return ns;
+}
+
+void window_pair_slots(int wl, int wt, int* a, int* b)
+{
+    with_window_pair(wl, wt, [&](auto NA_, auto NB_) {
+        *a = decltype(NA_)::value;
+        *b = decltype(NB_)::value;
+    });
 }
 
 int launch_sweep(const Level& L, Prec prec, int part, const void* r, bool r_f64, void* x, void* t, hipStream_t s,

@@ -37,6 +37,11 @@ int launch_fold(int64_t nparts, const RedArgs& ra, hipStream_t s, const int* gat
 int launch_spmv(const Level& L, Prec prec, const void* x, void* y, hipStream_t s, const int* gate);
 int launch_spmv_f64x(const Level& L, Prec prec, const double* x, double* y, const RedArgs* ra, hipStream_t s,
                      const int* gate);
+// slots a sweep loads per row for a window of w slots (its compile-time
+// bucket), and for a (lower, last-colour) pair of the post-smoother's forward
+// sweep
+int window_slots(int w);
+void window_pair_slots(int wl, int wt, int* a, int* b);
 // one sweep of a level: 0 zero-guess fwd, 1 zero-guess bwd, 2 fwd keeping t, 3 bwd from t,
 // 4 = 1 also keeping the upper sums outside the last colour in t, 5 = 2 reusing them
 int launch_sweep(const Level& L, Prec prec, int part, const void* r, bool r_f64, void* x, void* t, hipStream_t s,

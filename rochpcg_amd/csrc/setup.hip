@@ -174,7 +174,10 @@ __device__ __forceinline__ void cswap(T& a, T& b)
 // Build multicolour row r of the column-major ELL: its natural row i =
 // iperm[r]; the (up to) 27 neighbours are mapped through perm and sorted
 // ascending (odd-even transposition network over statically indexed
-// registers), padding (-1, 0.0) at the tail.
+// registers).  Split padding: lower entries and the diagonal from slot 0, the
+// upper entries right-aligned to slot W-1, padding (-1, 0.0) in between; the
+// descriptor gives the diagonal slot, the first last-colour slot and the
+// first upper slot (W when absent).
 template <typename VT>
 __global__ __launch_bounds__(256) void k_build_ell(Grid g, int64_t m, const int32_t* __restrict__ perm,
                                                    const int32_t* __restrict__ iperm,
@@ -203,22 +206,42 @@ __global__ __launch_bounds__(256) void k_build_ell(Grid g, int64_t m, const int3
 #pragma unroll
         for(int k = (round & 1); k + 1 < W; k += 2) cswap(key[k], key[k + 1]);
     }
-    int dg = 0, tl = W, ne = 0;
+    // sorted: [0, ne) valid, the diagonal at dg, the off-diagonal last-colour
+    // columns from tl0
+    int dg = 0, ne = 0, tl0 = W;
+#pragma unroll
+    for(int p = 0; p < W; ++p)
+    {
+        const int c = key[p];
+        if(c != INT_MAX) ne = p + 1;
+        if((int64_t)c == r) dg = p;
+        if(c != INT_MAX && c >= last_off && (int64_t)c != r && p < tl0) tl0 = p;
+    }
+    if(tl0 > ne) tl0 = ne;
+    // split padding: shift the upper entries right by W - ne (a barrel shift
+    // over statically indexed registers), leaving [dg+1, dg+1+shift) empty
+    const int shift = W - ne;
+#pragma unroll
+    for(int k = 16; k >= 1; k >>= 1)
+    {
+        const bool on = (shift & k) != 0;
+#pragma unroll
+        for(int p = W - 1; p >= 0; --p)
+        {
+            const int v = (p - k >= 0 && p - k > dg) ? key[p - k >= 0 ? p - k : 0] : (p > dg ? INT_MAX : key[p]);
+            if(on) key[p] = v;
+        }
+    }
 #pragma unroll
     for(int p = 0; p < W; ++p)
     {
         const int c = key[p];
         const bool valid = (c != INT_MAX);
-        const bool isd = valid && (int64_t)c == r;
         ell_col[(int64_t)p * m + r] = valid ? c : -1;
-        ell_val[(int64_t)p * m + r] = (VT)(valid ? (isd ? 26.0f : -1.0f) : 0.0f);
-        if(isd) dg = p;
-        if(valid) ne = p + 1;
-        if(valid && c >= last_off && p < tl) tl = p;
+        ell_val[(int64_t)p * m + r] = (VT)(valid ? (p == dg ? 26.0f : -1.0f) : 0.0f);
     }
-    if(tl > ne) tl = ne;
     diag_idx[r] = dg;
-    desc[r] = dg | (tl << 8) | (ne << 16);
+    desc[r] = dg | ((tl0 + shift) << 8) | ((dg + 1 + shift) << 16);
     if(inv_diag) inv_diag[r] = 1.0 / 26.0;
     dvals[r] = (VT)26.0f;
 }
@@ -449,97 +472,55 @@ int color_level(Level& L, hipStream_t s)
     return 0;
 }
 
-// Window statistics of one colour's rows [off, off+size): maxima of the three
-// windows for one row per thread and for row pairs (off+2j, off+2j+1), and the
-// window sums for one row per thread.  stats[0..5] (int max), sums[0..2].
+// Window statistics of one colour's rows [off, off+size): the largest lower
+// count d, upper count W - us and last-colour count W - tl (stats[0..2]).
 __global__ __launch_bounds__(256) void k_window_stats(int64_t off, int64_t size, const int32_t* __restrict__ desc,
-                                                      int* __restrict__ stats, unsigned long long* __restrict__ sums)
+                                                      int* __restrict__ stats)
 {
-    int mx[6] = {0, 0, 0, 0, 0, 0};
-    unsigned long long sm[3] = {0, 0, 0};
+    int mx[WIN_COUNT] = {0, 0, 0};
     for(int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < size; i += (int64_t)gridDim.x * 256)
     {
         const int v = desc[off + i];
-        const int d = v & 255, tl = (v >> 8) & 255, e = (v >> 16) & 255;
-        const int w1[3] = {d, e - (d + 1), e - tl};
-        for(int k = 0; k < 3; ++k)
-        {
-            mx[k] = max(mx[k], w1[k]);
-            sm[k] += (unsigned long long)max(w1[k], 0);
-        }
-        if((i & 1) == 0)
-        {
-            int maxd = d, maxe = e, mind1 = d + 1, mintl = tl;
-            if(i + 1 < size)
-            {
-                const int u = desc[off + i + 1];
-                const int d2 = u & 255, tl2 = (u >> 8) & 255, e2 = (u >> 16) & 255;
-                maxd = max(maxd, d2);
-                maxe = max(maxe, e2);
-                mind1 = min(mind1, d2 + 1);
-                mintl = min(mintl, tl2);
-            }
-            mx[3] = max(mx[3], maxd);
-            mx[4] = max(mx[4], maxe - mind1);
-            mx[5] = max(mx[5], maxe - mintl);
-        }
+        mx[WIN_LOWER] = max(mx[WIN_LOWER], v & 255);
+        mx[WIN_TAIL] = max(mx[WIN_TAIL], W - ((v >> 8) & 255));
+        mx[WIN_UPPER] = max(mx[WIN_UPPER], W - ((v >> 16) & 255));
     }
-    __shared__ int lmx[6][4];
-    __shared__ unsigned long long lsm[3][4];
+    __shared__ int lmx[WIN_COUNT][4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for(int k = 0; k < 6; ++k)
+    for(int k = 0; k < WIN_COUNT; ++k)
     {
         int v = mx[k];
         for(int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
         if(lane == 0) lmx[k][wid] = v;
     }
-    for(int k = 0; k < 3; ++k)
-    {
-        unsigned long long v = sm[k];
-        for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if(lane == 0) lsm[k][wid] = v;
-    }
     __syncthreads();
     if(threadIdx.x == 0)
-    {
-        for(int k = 0; k < 6; ++k) atomicMax(stats + k, max(max(lmx[k][0], lmx[k][1]), max(lmx[k][2], lmx[k][3])));
-        for(int k = 0; k < 3; ++k) atomicAdd(sums + k, lsm[k][0] + lsm[k][1] + lsm[k][2] + lsm[k][3]);
-    }
+        for(int k = 0; k < WIN_COUNT; ++k)
+            atomicMax(stats + k, max(max(lmx[k][0], lmx[k][1]), max(lmx[k][2], lmx[k][3])));
 }
 
-// Per colour and rows-per-thread, the widest slot window a sweep thread needs
-// (Level::win) and the window sums (Level::slots), from the row descriptors.
+// Per colour, the slots a sweep thread loads per row of each window kind
+// (Level::win), from the row descriptors.
 static int window_stats(Level& L, hipStream_t s)
 {
     const int nc = L.ncolors;
     int* stats = nullptr;
-    unsigned long long* sums = nullptr;
-    HPCGX_HIP(hipMalloc(&stats, sizeof(int) * 6 * nc));
-    HPCGX_HIP(hipMalloc(&sums, sizeof(unsigned long long) * 3 * nc));
-    HPCGX_HIP(hipMemsetAsync(stats, 0, sizeof(int) * 6 * nc, s));
-    HPCGX_HIP(hipMemsetAsync(sums, 0, sizeof(unsigned long long) * 3 * nc, s));
+    HPCGX_HIP(hipMalloc(&stats, sizeof(int) * WIN_COUNT * nc));
+    HPCGX_HIP(hipMemsetAsync(stats, 0, sizeof(int) * WIN_COUNT * nc, s));
     for(int c = 0; c < nc; ++c)
     {
         if(L.sizes[c] == 0) continue;
         const int64_t g = std::min<int64_t>(1024, ceil_div(L.sizes[c], 256));
         hipLaunchKernelGGL(k_window_stats, dim3((unsigned)g), dim3(256), 0, s, (int64_t)L.offsets[c],
-                           (int64_t)L.sizes[c], L.desc, stats + 6 * c, sums + 3 * c);
+                           (int64_t)L.sizes[c], L.desc, stats + WIN_COUNT * c);
         HPCGX_HIP(hipGetLastError());
     }
-    std::vector<int> hs((size_t)6 * nc);
-    std::vector<unsigned long long> hu((size_t)3 * nc);
-    HPCGX_HIP(hipMemcpyAsync(hs.data(), stats, sizeof(int) * 6 * nc, hipMemcpyDeviceToHost, s));
-    HPCGX_HIP(hipMemcpyAsync(hu.data(), sums, sizeof(unsigned long long) * 3 * nc, hipMemcpyDeviceToHost, s));
+    std::vector<int> hs((size_t)WIN_COUNT * nc);
+    HPCGX_HIP(hipMemcpyAsync(hs.data(), stats, sizeof(int) * WIN_COUNT * nc, hipMemcpyDeviceToHost, s));
     HPCGX_HIP(hipStreamSynchronize(s));
     HPCGX_HIP(hipFree(stats));
-    HPCGX_HIP(hipFree(sums));
     for(int c = 0; c < nc; ++c)
-        for(int k = 0; k < WIN_COUNT; ++k)
-        {
-            L.win[k][0][c] = (unsigned char)hs[6 * c + k];
-            L.win[k][1][c] = (unsigned char)hs[6 * c + 3 + k];
-            L.slots[k][c] = (int64_t)hu[3 * c + k];
-        }
+        for(int k = 0; k < WIN_COUNT; ++k) L.win[k][c] = (unsigned char)hs[WIN_COUNT * c + k];
     return 0;
 }
 

@@ -468,9 +468,9 @@ int cg_solve(Problem& P, int mode, bool precond, const double* b, double* x, int
 }
 
 // Algorithmic HBM bytes of one V-cycle from level l down, as this library's
-// kernels run it: every slot a kernel reads once (4 B column + value),
-// padding slots read only by the whole-row kernels, neighbour x gathers served
-// from cache, own-row vectors read and written once.
+// kernels run it: every slot a kernel loads once (4 B column + value; a sweep
+// loads its colour's window bucket for each row), neighbour x
+// gathers served from cache, own-row vectors read and written once.
 static double vcycle_bytes(const Problem& P, int l, bool mixed)
 {
     const Level& L = P.L[l];
@@ -484,8 +484,8 @@ static double vcycle_bytes(const Problem& P, int l, bool mixed)
     for(int c = 0; c <= last; ++c)
     {
         const double n = L.sizes[c];
-        b += L.slots[WIN_LOWER][c] * slot + n * (4 + vb + rb + xb);   // fwd_zero
-        if(c < last) b += L.slots[WIN_UPPER][c] * slot + n * (4 + vb + 2 * xb + (keep_u ? xb : 0));   // bwd_zero
+        b += window_slots(L.win[WIN_LOWER][c]) * n * slot + n * (4 + vb + rb + xb);   // fwd_zero
+        if(c < last) b += window_slots(L.win[WIN_UPPER][c]) * n * slot + n * (4 + vb + 2 * xb + (keep_u ? xb : 0));   // bwd_zero
     }
     if(coarsest) return b;
     const double mc = (double)L.mc;
@@ -496,10 +496,14 @@ static double vcycle_bytes(const Problem& P, int l, bool mixed)
     {
         const double n = L.sizes[c];
         if(c < last && keep_u)
-            b += (L.slots[WIN_LOWER][c] + L.slots[WIN_TAIL][c]) * slot + n * (4 + vb + rb + 3 * xb);
+        {
+            int na = 0, nb = 0;
+            window_pair_slots(L.win[WIN_LOWER][c], L.win[WIN_TAIL][c], &na, &nb);
+            b += (na + nb) * n * slot + n * (4 + vb + rb + 3 * xb);
+        }
         else
             b += n * (W * slot + rb + xb + (c < last ? xb : 0));   // whole-row forward sweep
-        if(c < last) b += L.slots[WIN_UPPER][c] * slot + n * (4 + vb + 2 * xb);   // bwd_full
+        if(c < last) b += window_slots(L.win[WIN_UPPER][c]) * n * slot + n * (4 + vb + 2 * xb);   // bwd_full
     }
     return b;
 }

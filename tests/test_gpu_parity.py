@@ -42,21 +42,36 @@ def test_setup_bit_exact(hip, oracle_mod, dims):
         sizes, offsets, _ = G.color_table(lv)
         assert np.array_equal(sizes, o["sizes"]) and np.array_equal(offsets, o["offsets"])
         cols = g["ell_col"].reshape(27, m).T
-        assert np.array_equal(cols, o["cols"]), f"ell_col level {lv}"
-        assert np.array_equal(g["ell_val"].reshape(27, m).T, o["vals"]), f"ell_val level {lv}"
-        assert np.array_equal(g["ell_f16"].reshape(27, m).T.astype(np.float64), o["vals"])
+        vals = g["ell_val"].reshape(27, m).T
+        # split padding: the reference's entries in the reference's order ...
+        keep = np.argsort(cols < 0, axis=1, kind="stable")
+        assert np.array_equal(np.take_along_axis(cols, keep, 1), o["cols"]), f"ell_col level {lv}"
+        assert np.array_equal(np.take_along_axis(vals, keep, 1), o["vals"]), f"ell_val level {lv}"
+        f16 = g["ell_f16"].reshape(27, m).T.astype(np.float64)
+        assert np.array_equal(f16, vals), f"ell_f16 level {lv}"
         assert np.array_equal(g["diag_idx"], o["diag"])
-        # row descriptor: diagonal slot, first slot holding a column of the
-        # last colour (end of row if none), number of valid slots
+        # ... with the lower entries and the diagonal from slot 0 and the upper
+        # entries ending at slot 26; padding (-1, 0.0) in between
         oc = o["cols"]
-        valid = oc >= 0
-        ne = valid.sum(axis=1)
-        in_last = valid & (oc >= offsets[L.ncolors - 1])
-        tl = np.where(in_last.any(axis=1), np.argmax(in_last, axis=1), ne)
+        ne = (oc >= 0).sum(axis=1)
+        d = o["diag"]
+        us = 27 - (ne - d - 1)
+        slot = np.arange(27)[None, :]
+        used = (slot <= d[:, None]) | (slot >= us[:, None])
+        assert np.array_equal(cols >= 0, used), f"ell padding level {lv}"
+        assert np.all(vals[~used] == 0.0)
+        # row descriptor: diagonal slot, first slot of an off-diagonal
+        # last-colour column, first upper slot (27 when there is none)
+        nlast = ((oc >= offsets[L.ncolors - 1]) & (slot != d[:, None])).sum(axis=1)
         desc = g["desc"]
-        assert np.array_equal(desc & 255, o["diag"]), f"desc diag level {lv}"
-        assert np.array_equal((desc >> 8) & 255, tl), f"desc tail level {lv}"
-        assert np.array_equal(desc >> 16, ne), f"desc end level {lv}"
+        assert np.array_equal(desc & 255, d), f"desc diag level {lv}"
+        assert np.array_equal((desc >> 8) & 255, 27 - nlast), f"desc tail level {lv}"
+        assert np.array_equal(desc >> 16, us), f"desc upper level {lv}"
+        tail = slot >= (27 - nlast)[:, None]
+        off_diag = slot != d[:, None]
+        assert np.all(cols[tail] >= offsets[L.ncolors - 1])
+        assert np.all(cols[used & ~tail & off_diag] < offsets[L.ncolors - 1])
+        assert np.array_equal(g["diag"], np.full(m, 26.0)), f"diagonal array level {lv}"
         assert np.all(g["inv_diag"] == 1.0 / 26.0)
         if lv + 1 < 4:
             # oracle f2c is indexed by the coarse multicolour row
@@ -82,7 +97,9 @@ def test_setup_equal_hash_level(hip, oracle_mod):
     O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
     g, o = G.level_arrays(3), O.export(3)
     assert np.array_equal(g["colors"], o["colors"])
-    assert np.array_equal(g["ell_col"].reshape(27, -1).T, o["cols"])
+    cols = g["ell_col"].reshape(27, -1).T
+    keep = np.argsort(cols < 0, axis=1, kind="stable")
+    assert np.array_equal(np.take_along_axis(cols, keep, 1), o["cols"])
 
 
 @pytest.mark.parametrize("dims", SHAPES[:3])
