@@ -268,7 +268,7 @@ __device__ __forceinline__ void row_spmv(int64_t m, int64_t row, const int32_t* 
 // [us, W) the upper entries, right-aligned, and the last-colour entries (the
 // largest columns) are their tail [tl, W).  Every window a sweep reads then
 // starts at a slot that does not depend on the row -- 0 for the lower
-// entries, W - n for the upper or last-colour ones -- so its loads are issued
+// entries, W - NS for the upper or last-colour ones -- so its loads are issued
 // with the row's descriptor load instead of after it.
 //
 // Row descriptor, one int32 per row: bits 0-7 the diagonal slot d, bits 8-15
