@@ -98,12 +98,13 @@ struct Level
     int offsets[MAX_COLORS + 1] = {};
     int round_sizes[MAX_COLORS] = {};   // per JPL round, as the reference records them
 
-    // column-major ELL: entry p of row r lives at p*m + r (64-bit offsets)
-    int32_t* ell_col = nullptr;   // [W*m], -1 padding at the tail of each row
+    // column-major ELL: entry p of row r lives at p*m + r (64-bit offsets); the
+    // entries of a row in ascending column order, split-padded (kernels.hpp)
+    int32_t* ell_col = nullptr;   // [W*m], -1 padding between the diagonal and the right-aligned upper entries
     double* ell_val = nullptr;    // [W*m] (f64 levels)
     __half* ell_h = nullptr;      // [W*m] (mixed levels)
     int32_t* diag_idx = nullptr;  // [m] slot of the diagonal
-    int32_t* desc = nullptr;      // [m] row descriptor: diagonal slot, first last-colour slot, valid slots
+    int32_t* desc = nullptr;      // [m] row descriptor: diagonal slot, first last-colour slot, first upper slot
     double* inv_diag = nullptr;   // [m] 1/a_ii (f64)
     double* diag64 = nullptr;     // [m] a_ii (f64 levels)
     __half* diag_h = nullptr;     // [m] a_ii (mixed levels)
