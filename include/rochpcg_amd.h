@@ -134,7 +134,9 @@ int hpcgx_cg(hpcgx_problem* p, int mode, const double* b, double* x, int max_ite
              int precond, int use_graph, int* niters, double* normr, double* normr0,
              double* history, int hist_len, double* times);
 /* the same loop in pieces, for timing: preamble; `count` iterations enqueued
- * (no synchronisation); results */
+ * (no synchronisation); results.  Each iteration's x += alpha p is applied by
+ * the next one, so x is complete after hpcgx_cg_end (which applies the last)
+ * and lags one update in between. */
 int hpcgx_cg_begin(hpcgx_problem* p, int mode, const double* b, double* x, int max_iter,
                    double tolerance, int precond);
 int hpcgx_cg_iterate(hpcgx_problem* p, int count, int use_graph);

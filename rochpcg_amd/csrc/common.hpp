@@ -141,7 +141,7 @@ struct CGScalars
     int iter;       // iterations completed
     int done;       // 1 once normr/normr0 <= tol or max_iter reached
     int max_iter;
-    int pad;
+    int xpend;      // single-problem CG: x still lacks the last iteration's alpha*p
 };
 
 // Deterministic two-phase reductions: every block of the producing kernel

@@ -50,11 +50,13 @@ __device__ inline void apply_op(int op, double v, CGScalars* sc, double* hist, i
     case OP_PAP:
         sc->pAp = v;
         sc->alpha = sc->rtz / v;
+        sc->xpend = 0;   // the p update before this SpMV applied the pending x update
         break;
     case OP_NORMR:
     {
         const double nr = sqrt(v);
         sc->normr = nr;
+        sc->xpend = 1;   // x += alpha*p waits for the next p update (or the end)
         const int it = sc->iter + 1;
         sc->iter = it;
         if(hist && it <= hist_cap) hist[it] = nr;
@@ -65,6 +67,7 @@ __device__ inline void apply_op(int op, double v, CGScalars* sc, double* hist, i
     {
         const double nr = sqrt(v);
         sc->normr = nr;
+        sc->xpend = 0;   // the true residual was taken from an updated x
         const int it = sc->iter;
         if(hist && it <= hist_cap) hist[it] = nr;
         sc->done = (!(nr / sc->normr0 > sc->tol) || it >= sc->max_iter) ? 1 : 0;

@@ -247,22 +247,37 @@ __global__ __launch_bounds__(BS) void k_axpy_dot(int64_t n, double alpha, const 
 // p = z (first iteration) or p = z + beta*p, with beta read on the device
 template <typename ZT, int BS>
 __global__ __launch_bounds__(BS) void k_cg_p(int64_t n, const ZT* __restrict__ z, double* __restrict__ p,
-                                             const CGScalars* __restrict__ sc, const int* gate)
+                                             double* __restrict__ x, const CGScalars* __restrict__ sc,
+                                             const int* gate)
 {
+    // p = z + beta*p (CG.cpp:158-165), and first the previous iteration's
+    // deferred x += alpha*p, which reads the p this kernel overwrites anyway
     if(gated(gate)) return;
     int64_t b, e;
     chunk_range(n, b, e);
     const bool first = sc->iter == 0;
+    const bool xpend = sc->xpend != 0;
     const double beta = sc->beta;
+    const double alpha = sc->alpha;
     for(int64_t i = b + threadIdx.x; i < e; i += BS)
-        p[i] = first ? (double)z[i] : fmaT(beta, p[i], (double)z[i]);
+    {
+        if(first)
+            p[i] = (double)z[i];
+        else
+        {
+            const double pv = p[i];
+            if(xpend) x[i] = fmaT(alpha, pv, x[i]);
+            p[i] = fmaT(beta, pv, (double)z[i]);
+        }
+    }
 }
 
-// x += alpha*p; r -= alpha*Ap; ||r||^2 -> normr, with alpha read on the device
+// r -= alpha*Ap; ||r||^2 -> normr, with alpha read on the device.  The
+// matching x += alpha*p is deferred to the next p update (k_cg_p) or to
+// k_cg_xflush, saving one pass over p.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_cg_xr(int64_t n, const double* __restrict__ p, const double* __restrict__ Ap,
-                                              double* __restrict__ x, double* __restrict__ r, RedArgs ra,
-                                              const int* gate)
+__global__ __launch_bounds__(BS) void k_cg_xr(int64_t n, const double* __restrict__ Ap, double* __restrict__ r,
+                                              RedArgs ra, const int* gate)
 {
     if(gated(gate)) return;
     int64_t b, e;
@@ -273,10 +288,21 @@ __global__ __launch_bounds__(BS) void k_cg_xr(int64_t n, const double* __restric
     {
         const double rv = fmaT(-alpha, Ap[i], r[i]);
         r[i] = rv;
-        x[i] = fmaT(alpha, p[i], x[i]);
         acc = fmaT(rv, rv, acc);
     }
     block_partial<BS>(acc, ra.partial, blockIdx.x);
+}
+
+// x += alpha*p if an iteration's x update is still pending (sc->xpend)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_cg_xflush(int64_t n, const double* __restrict__ p, double* __restrict__ x,
+                                                  const CGScalars* __restrict__ sc, const int* gate)
+{
+    if(gated(gate) || sc->xpend == 0) return;
+    const double alpha = sc->alpha;
+    int64_t b, e;
+    chunk_range(n, b, e);
+    for(int64_t i = b + threadIdx.x; i < e; i += BS) x[i] = fmaT(alpha, p[i], x[i]);
 }
 
 template <typename T, int BS>
@@ -698,25 +724,35 @@ int launch_axpy_dot(int64_t n, double alpha, const double* x, double* y, const R
     return launch_fold(g, ra, s, gate);
 }
 
-int launch_cg_p(int64_t n, const void* z, bool z_f64, double* p, const CGScalars* sc, hipStream_t s, const int* gate)
+int launch_cg_p(int64_t n, const void* z, bool z_f64, double* p, double* x, const CGScalars* sc, hipStream_t s,
+                const int* gate)
 {
     constexpr int BS = RED_THREADS;
     if(z_f64)
-        hipLaunchKernelGGL((k_cg_p<double, BS>), dim3(red_grid(n)), dim3(BS), 0, s, n, (const double*)z, p, sc, gate);
+        hipLaunchKernelGGL((k_cg_p<double, BS>), dim3(red_grid(n)), dim3(BS), 0, s, n, (const double*)z, p, x, sc,
+                           gate);
     else
-        hipLaunchKernelGGL((k_cg_p<float, BS>), dim3(red_grid(n)), dim3(BS), 0, s, n, (const float*)z, p, sc, gate);
+        hipLaunchKernelGGL((k_cg_p<float, BS>), dim3(red_grid(n)), dim3(BS), 0, s, n, (const float*)z, p, x, sc,
+                           gate);
     HPCGX_KCHECK();
     return 0;
 }
 
-int launch_cg_xr(int64_t n, const double* p, const double* Ap, double* x, double* r, const RedArgs& ra,
-                 hipStream_t s, const int* gate)
+int launch_cg_xr(int64_t n, const double* Ap, double* r, const RedArgs& ra, hipStream_t s, const int* gate)
 {
     constexpr int BS = RED_THREADS;
     const unsigned g = red_grid(n);
-    hipLaunchKernelGGL((k_cg_xr<BS>), dim3(g), dim3(BS), 0, s, n, p, Ap, x, r, ra, gate);
+    hipLaunchKernelGGL((k_cg_xr<BS>), dim3(g), dim3(BS), 0, s, n, Ap, r, ra, gate);
     HPCGX_KCHECK();
     return launch_fold(g, ra, s, gate);
+}
+
+int launch_cg_xflush(int64_t n, const double* p, double* x, const CGScalars* sc, hipStream_t s, const int* gate)
+{
+    constexpr int BS = RED_THREADS;
+    hipLaunchKernelGGL((k_cg_xflush<BS>), dim3(red_grid(n)), dim3(BS), 0, s, n, p, x, sc, gate);
+    HPCGX_KCHECK();
+    return 0;
 }
 
 int launch_fill(int64_t n, void* p, bool f64, double v, hipStream_t s, const int* gate)

@@ -64,10 +64,14 @@ int launch_waxpby(int64_t n, double alpha, const double* x, double beta, const d
                   const int* gate);
 int launch_axpy_dot(int64_t n, double alpha, const double* x, double* y, const RedArgs& ra, hipStream_t s,
                     const int* gate);
-int launch_cg_p(int64_t n, const void* z, bool z_f64, double* p, const CGScalars* sc, hipStream_t s,
+// p = z + beta p, applying first the x += alpha p that the previous
+// launch_cg_xr left pending (CGScalars::xpend)
+int launch_cg_p(int64_t n, const void* z, bool z_f64, double* p, double* x, const CGScalars* sc, hipStream_t s,
                 const int* gate);
-int launch_cg_xr(int64_t n, const double* p, const double* Ap, double* x, double* r, const RedArgs& ra,
-                 hipStream_t s, const int* gate);
+// r -= alpha Ap and ||r||; x += alpha p stays pending
+int launch_cg_xr(int64_t n, const double* Ap, double* r, const RedArgs& ra, hipStream_t s, const int* gate);
+// apply a pending x += alpha p (before x is read: true residual, end of CG)
+int launch_cg_xflush(int64_t n, const double* p, double* x, const CGScalars* sc, hipStream_t s, const int* gate);
 int launch_fill(int64_t n, void* p, bool f64, double v, hipStream_t s, const int* gate);
 int launch_copy(int64_t n, const void* src, bool src_f64, void* dst, bool dst_f64, hipStream_t s, const int* gate);
 int launch_get_diag(const Level& L, double* d, hipStream_t s);

@@ -276,13 +276,14 @@ static int enqueue_iteration(Problem& P, bool replace)
     else
         HPCGX_TRY(launch_copy(m, P.r, true, z, !mixed, P.s, gate));
     HPCGX_TRY(launch_dot(m, P.r, true, z, !mixed, red(P, OP_RTZ), P.s, gate));
-    HPCGX_TRY(launch_cg_p(m, z, !mixed, P.p, P.sc, P.s, gate));
+    HPCGX_TRY(launch_cg_p(m, z, !mixed, P.p, P.cur_x, P.sc, P.s, gate));
     RedArgs rp = red(P, OP_PAP);
     HPCGX_TRY(launch_spmv_f64x(P.L[0], outer_prec(P), P.p, P.Ap, &rp, P.s, gate));
-    HPCGX_TRY(launch_cg_xr(m, P.p, P.Ap, P.cur_x, P.r, red(P, OP_NORMR), P.s, gate));
+    HPCGX_TRY(launch_cg_xr(m, P.Ap, P.r, red(P, OP_NORMR), P.s, gate));
     if(replace)
     {
         // true residual r = b - A x in f64 (iterative refinement of the recursive residual)
+        HPCGX_TRY(launch_cg_xflush(m, P.p, P.cur_x, P.sc, P.s, gate));
         HPCGX_TRY(launch_spmv_f64x(P.L[0], outer_prec(P), P.cur_x, P.tmp, nullptr, P.s, gate));
         HPCGX_TRY(launch_waxpby(m, 1.0, P.cur_b, -1.0, P.tmp, P.r, P.s, gate));
         HPCGX_TRY(launch_dot(m, P.r, true, P.r, true, red(P, OP_NORMR_SET), P.s, gate));
@@ -384,20 +385,21 @@ int cg_iterations_profiled(Problem& P, int count, double* phase_ms)
         (void)end(PH_DOT);
         if(rc) break;
         (void)beg(PH_WAXPBY);
-        rc = launch_cg_p(m, z, !mixed, P.p, P.sc, P.s, gate);
+        rc = launch_cg_p(m, z, !mixed, P.p, P.cur_x, P.sc, P.s, gate);
         if(rc) break;
         (void)beg(PH_SPMV);
         RedArgs rp = red(P, OP_PAP);
         rc = launch_spmv_f64x(P.L[0], outer_prec(P), P.p, P.Ap, &rp, P.s, gate);
         (void)end(PH_SPMV);
         if(rc) break;
-        rc = launch_cg_xr(m, P.p, P.Ap, P.cur_x, P.r, red(P, OP_NORMR), P.s, gate);
+        rc = launch_cg_xr(m, P.Ap, P.r, red(P, OP_NORMR), P.s, gate);
         (void)end(PH_WAXPBY);
         if(rc) break;
         if(replace_at(P, P.host_iter))
         {
             (void)beg(PH_REPLACE);
-            rc = launch_spmv_f64x(P.L[0], outer_prec(P), P.cur_x, P.tmp, nullptr, P.s, gate);
+            rc = launch_cg_xflush(m, P.p, P.cur_x, P.sc, P.s, gate);
+            if(!rc) rc = launch_spmv_f64x(P.L[0], outer_prec(P), P.cur_x, P.tmp, nullptr, P.s, gate);
             if(!rc) rc = launch_waxpby(m, 1.0, P.cur_b, -1.0, P.tmp, P.r, P.s, gate);
             if(!rc) rc = launch_dot(m, P.r, true, P.r, true, red(P, OP_NORMR_SET), P.s, gate);
             (void)end(PH_REPLACE);
@@ -430,6 +432,14 @@ int cg_iterations_profiled(Problem& P, int count, double* phase_ms)
 
 int cg_end(Problem& P, int* niters, double* normr, double* normr0, double* history, int hist_len)
 {
+    // the last iteration's x += alpha*p (deferred to the next p update, which
+    // did not come), once: clear the flag so that further iterations or a
+    // second cg_end do not apply it again
+    if(P.cur_x)
+    {
+        HPCGX_TRY(launch_cg_xflush(P.L[0].m, P.p, P.cur_x, P.sc, P.s, nullptr));
+        HPCGX_HIP(hipMemsetAsync(&P.sc->xpend, 0, sizeof(int), P.s));
+    }
     HPCGX_HIP(hipStreamSynchronize(P.s));
     CGScalars h;
     HPCGX_HIP(hipMemcpy(&h, P.sc, sizeof(h), hipMemcpyDeviceToHost));
@@ -517,8 +527,8 @@ int iteration_bytes(const Problem& P, int mode, double* vcycle, double* iteratio
     double it = mg;
     it += m0 * (W * (4 + vb) + 16);            // SpMV + p'Ap
     it += m0 * (8 + zb);                       // r'z
-    it += m0 * (zb + 16);                      // p = z + beta p
-    it += m0 * 48;                             // x, r updates and ||r||
+    it += m0 * (zb + 32);                      // x += alpha p (deferred), p = z + beta p
+    it += m0 * 24;                             // r update and ||r||
     if(mixed && P.replace_every > 0)           // true residual every k iterations
         it += m0 * (W * (4 + vb) + 16 + 24 + 16) / P.replace_every;
     if(vcycle) *vcycle = mg;
