@@ -122,3 +122,14 @@ def test_benchmark_driver_report(tmp_path, extra):
     calls = [ln for ln in logs[0].read_text().splitlines() if ln.startswith("Call [")]
     assert len(calls) == int(y["Iteration Count Information/Total number of optimized iterations"]) // opt
     assert calls[0].startswith("Call [0] Scaled Residual [")
+
+
+@pytest.mark.gpu
+def test_benchmark_driver_rejects_missing_device(tmp_path):
+    """--dev= selects the GPU before setup (init.cpp:193-214); an id the
+    machine does not have is an error before any device work."""
+    args = [XHPCG, "--nx=16", "--ny=16", "--nz=16", "--rt=0", "--quick", "--dev=64"]
+    print(f"[gpu] {' '.join(args)}", flush=True)
+    out = subprocess.run(args, capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert out.returncode != 0
+    assert "invalid device ID 64" in out.stderr
