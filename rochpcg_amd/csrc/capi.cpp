@@ -381,9 +381,14 @@ int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps,
         switch(op)
         {
         case 0:
-            if(mixed) return launch_spmv_f64x(L, PREC_MIXED, P->p, P->Ap, nullptr, P->s, nullptr);
+        {
+            // level 0 runs as the CG runs it: with the fused p'Ap partials and their fold
+            RedArgs ra = store_red(P);
+            const RedArgs* dot = (level == 0) ? &ra : nullptr;
+            if(mixed) return launch_spmv_f64x(L, PREC_MIXED, P->p, P->Ap, dot, P->s, nullptr);
             return launch_spmv_f64x(L, PREC_F64, (level == 0) ? P->p : (const double*)V[level].r,
-                                    (level == 0) ? P->Ap : (double*)V[level].x, nullptr, P->s, nullptr);
+                                    (level == 0) ? P->Ap : (double*)V[level].x, dot, P->s, nullptr);
+        }
         case 1: return launch_symgs(L, pr, r, r_f64, x, V[level].t, P->s, nullptr);
         case 2: return launch_symgs_zero(L, pr, r, r_f64, x, P->s, nullptr);
         case 3: return mg(*P, level, pr, r, r_f64, x, nullptr);
