@@ -77,25 +77,11 @@ def batched_leg(hip, count, log):
     for k, P in enumerate(B.problems):
         P.make_rhs(1, 7 + k)
     flops = np.array([vv.hpcg_flops_per_iteration(d) for d in dims])
-    # block size of the batched kernels: pick the fastest of a short FP64 run
-    best = None
-    for block in (64, 128, 256):
-        for lv in range(4):
-            B.set_block(lv, block)
-        for P in B.problems:
-            P.vec["x"].upload(np.zeros(P.m))
-        B.cg(max_iter=3)   # graph capture
-        for P in B.problems:
-            P.vec["x"].upload(np.zeros(P.m))
-        t = time.perf_counter()
-        B.cg(max_iter=10)
-        el = time.perf_counter() - t
-        if best is None or el < best[0]:
-            best = (el, block)
-    for lv in range(4):
-        B.set_block(lv, best[1])
+    # threads per block of the batched kernels, per level (both CG modes)
+    from rochpcg_amd import autotune
+    blocks = autotune.tune_batch(B, 4, (hip.CG_F64, hip.CG_MIXED), log=log)
     res = {"problems": count, "rows_total": int(sum(a * b * c for a, b, c in dims)),
-           "setup_s": round(setup, 3), "block": best[1]}
+           "setup_s": round(setup, 3), "block": blocks}
     for name, mode in (("f64", hip.CG_F64), ("mixed", hip.CG_MIXED)):
         for P in B.problems:
             P.vec["x"].upload(np.zeros(P.m))

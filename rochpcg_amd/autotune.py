@@ -113,3 +113,54 @@ def tune(prob, mixed=False, reps=3, rounds=3, levels=None, table=None, log=None)
                 log(f"tune {_key(L, mixed)} {OP_NAMES[op]:9s} best block={block} rows={rows} "
                     f"{best * 1e3:9.1f} us (256x1: {default * 1e3:9.1f} us)")
     return table
+
+
+def _batch_seconds(B, mode, iters):
+    """Seconds of `iters` graph-replayed batched CG iterations from x = 0 (the
+    graph captured beforehand)."""
+    import time
+
+    import numpy as np
+
+    for P in B.problems:
+        P.vec["x"].upload(np.zeros(P.m))
+    B.cg(max_iter=2, mode=mode)
+    for P in B.problems:
+        P.vec["x"].upload(np.zeros(P.m))
+    t = time.perf_counter()
+    B.cg(max_iter=iters, mode=mode)
+    return time.perf_counter() - t
+
+
+def tune_batch(B, nlevels, modes, iters=10, blocks=(64, 128, 256), log=None):
+    """Threads per block of the batched kernels, per level: the best uniform
+    size first, then each level in turn keeps the size that minimises the
+    summed time of `iters` iterations in every mode of `modes`.  Returns the
+    per-level sizes (already applied)."""
+    def cost():
+        return sum(_batch_seconds(B, m, iters) for m in modes)
+
+    def apply(sizes):
+        for lv, b in enumerate(sizes):
+            B.set_block(lv, b)
+
+    best = None
+    for b in blocks:
+        apply([b] * nlevels)
+        t = cost()
+        if best is None or t < best[0]:
+            best = (t, [b] * nlevels)
+    for lv in range(nlevels):
+        for b in blocks:
+            if b == best[1][lv]:
+                continue
+            trial = list(best[1])
+            trial[lv] = b
+            apply(trial)
+            t = cost()
+            if t < best[0]:
+                best = (t, trial)
+    apply(best[1])
+    if log:
+        log(f"batch blocks per level {best[1]} ({best[0] * 1e3:.1f} ms for {iters} iterations per mode)")
+    return best[1]
