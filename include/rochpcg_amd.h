@@ -174,6 +174,10 @@ int hpcgx_batch_destroy(hpcgx_batch* b);
 int hpcgx_batch_problem(hpcgx_batch* b, int k, hpcgx_problem** out);
 /* threads per block of every batched kernel on `level` (64, 128 or 256) */
 int hpcgx_batch_set_block(hpcgx_batch* b, int level, int block);
+/* rows per thread (1 or 2) of a level's batched sweep launches; a launch whose
+ * segments are not all even in offset, size and row count runs with 1 (no reference
+ * counterpart; the autotuner's knob) */
+int hpcgx_batch_set_rows(hpcgx_batch* b, int level, int rows);
 /* CG on every problem from its own x (b, x of each problem); niters, normr,
  * normr0: arrays of nprob entries (may be NULL) */
 int hpcgx_batch_cg(hpcgx_batch* b, int mode, int max_iter, double tolerance, int use_graph, int* niters,

@@ -82,6 +82,7 @@ def _load():
         "hpcgx_batch_destroy": ([vp], i32),
         "hpcgx_batch_problem": ([vp, i32, P(vp)], i32),
         "hpcgx_batch_set_block": ([vp, i32, i32], i32),
+        "hpcgx_batch_set_rows": ([vp, i32, i32], i32),
         "hpcgx_batch_cg": ([vp, i32, i32, f64, i32, vp, vp, vp], i32),
         "hpcgx_malloc": ([P(vp), C.c_size_t], i32),
         "hpcgx_free": ([vp], i32),
@@ -401,6 +402,9 @@ class Batch:
 
     def set_block(self, level, block):
         _check(lib.hpcgx_batch_set_block(self.h, level, block))
+
+    def set_rows(self, level, rows):
+        _check(lib.hpcgx_batch_set_rows(self.h, level, rows))
 
     def cg(self, max_iter=50, tol=0.0, mode=CG_F64, use_graph=True):
         n = len(self.problems)

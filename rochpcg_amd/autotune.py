@@ -132,35 +132,39 @@ def _batch_seconds(B, mode, iters):
     return time.perf_counter() - t
 
 
-def tune_batch(B, nlevels, modes, iters=10, blocks=(64, 128, 256), log=None):
-    """Threads per block of the batched kernels, per level: the best uniform
-    size first, then each level in turn keeps the size that minimises the
-    summed time of `iters` iterations in every mode of `modes`.  Returns the
-    per-level sizes (already applied)."""
+def tune_batch(B, nlevels, modes, iters=10, blocks=(64, 128, 256), rows=(1, 2), log=None):
+    """Launch shape of the batched kernels, per level: threads per block (all
+    kernels) and rows per thread (sweeps).  The best uniform shape first, then
+    each level in turn keeps the shape that minimises the summed time of
+    `iters` iterations in every mode of `modes`.  Returns the per-level
+    (block, rows) pairs (already applied)."""
+    shapes = [(b, r) for r in rows for b in blocks]
+
     def cost():
         return sum(_batch_seconds(B, m, iters) for m in modes)
 
-    def apply(sizes):
-        for lv, b in enumerate(sizes):
+    def apply(per_level):
+        for lv, (b, r) in enumerate(per_level):
             B.set_block(lv, b)
+            B.set_rows(lv, r)
 
     best = None
-    for b in blocks:
-        apply([b] * nlevels)
+    for sh in shapes:
+        apply([sh] * nlevels)
         t = cost()
         if best is None or t < best[0]:
-            best = (t, [b] * nlevels)
+            best = (t, [sh] * nlevels)
     for lv in range(nlevels):
-        for b in blocks:
-            if b == best[1][lv]:
+        for sh in shapes:
+            if sh == best[1][lv]:
                 continue
             trial = list(best[1])
-            trial[lv] = b
+            trial[lv] = sh
             apply(trial)
             t = cost()
             if t < best[0]:
                 best = (t, trial)
     apply(best[1])
     if log:
-        log(f"batch blocks per level {best[1]} ({best[0] * 1e3:.1f} ms for {iters} iterations per mode)")
+        log(f"batch (block, rows) per level {best[1]} ({best[0] * 1e3:.1f} ms for {iters} iterations per mode)")
     return best[1]

@@ -69,66 +69,67 @@ __global__ __launch_bounds__(BS) void kb_spmv(const BSeg* __restrict__ segs, int
 }
 
 // The sweeps are compiled for the widest window of the launch's segments
-// (BLaunch::win), like the single-problem kernels.
-template <int NS, typename VT, typename XT, typename RT, int BS>
+// (BLaunch::win), like the single-problem kernels, and for R rows per thread
+// (BLaunch::rows; 2 only when every segment's offset and size are even).
+template <int NS, typename VT, typename XT, typename RT, int R, int BS>
 __global__ __launch_bounds__(BS) void kb_fwd_zero(const BSeg* __restrict__ segs, int nseg)
 {
     BSEG();
-    const int64_t i = lb * BS + threadIdx.x;
+    const int64_t i = (lb * BS + threadIdx.x) * R;
     if(i >= S.size) return;
-    row_fwd_zero<NS, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+    row_fwd_zero<NS, VT, XT, RT, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
                                     (const RT*)S.r, (XT*)S.x);
 }
 
-template <int NS, typename VT, typename XT, int BS>
+template <int NS, typename VT, typename XT, int R, int BS>
 __global__ __launch_bounds__(BS) void kb_bwd_zero(const BSeg* __restrict__ segs, int nseg)
 {
     BSEG();
-    const int64_t i = lb * BS + threadIdx.x;
+    const int64_t i = (lb * BS + threadIdx.x) * R;
     if(i >= S.size) return;
     if(S.t)
-        row_bwd_zero<NS, VT, XT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+        row_bwd_zero<NS, VT, XT, true, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
                                           (XT*)S.x, (XT*)S.t);
     else
-        row_bwd_zero<NS, VT, XT, false, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+        row_bwd_zero<NS, VT, XT, false, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
                                            (XT*)S.x, nullptr);
 }
 
 // every segment reuses its pre-smoother's upper sums (BLaunch::all_u)
-template <int NA, int NB, typename VT, typename XT, typename RT, int BS>
+template <int NA, int NB, typename VT, typename XT, typename RT, int R, int BS>
 __global__ __launch_bounds__(BS) void kb_fwd_full_u(const BSeg* __restrict__ segs, int nseg)
 {
     BSEG();
-    const int64_t i = lb * BS + threadIdx.x;
+    const int64_t i = (lb * BS + threadIdx.x) * R;
     if(i >= S.size) return;
-    row_fwd_full_u<NA, NB, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg,
-                                          S.desc, (const RT*)S.r, (XT*)S.x, (XT*)S.t);
+    row_fwd_full_u<NA, NB, VT, XT, RT, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+                                          (const RT*)S.r, (XT*)S.x, (XT*)S.t);
 }
 
-template <typename VT, typename XT, typename RT, int BS>
+template <typename VT, typename XT, typename RT, int R, int BS>
 __global__ __launch_bounds__(BS) void kb_fwd_full(const BSeg* __restrict__ segs, int nseg)
 {
     BSEG();
-    const int64_t i = lb * BS + threadIdx.x;
+    const int64_t i = (lb * BS + threadIdx.x) * R;
     if(i >= S.size) return;
     if(S.use_u)
-        row_fwd_full_u<W, W, VT, XT, RT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg,
-                                            S.desc, (const RT*)S.r, (XT*)S.x, (XT*)S.t);
+        row_fwd_full_u<W - 1, W - 1, VT, XT, RT, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg,
+                                                    S.desc, (const RT*)S.r, (XT*)S.x, (XT*)S.t);
     else if(S.t)
-        row_fwd_full<VT, XT, RT, true, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
+        row_fwd_full<VT, XT, RT, true, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
                                           (XT*)S.t);
     else
-        row_fwd_full<VT, XT, RT, false, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
+        row_fwd_full<VT, XT, RT, false, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
                                            nullptr);
 }
 
-template <int NS, typename VT, typename XT, int BS>
+template <int NS, typename VT, typename XT, int R, int BS>
 __global__ __launch_bounds__(BS) void kb_bwd_full(const BSeg* __restrict__ segs, int nseg)
 {
     BSEG();
-    const int64_t i = lb * BS + threadIdx.x;
+    const int64_t i = (lb * BS + threadIdx.x) * R;
     if(i >= S.size) return;
-    row_bwd_full<NS, VT, XT, 1>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+    row_bwd_full<NS, VT, XT, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
                                 (const XT*)S.t, (XT*)S.x);
 }
 
@@ -231,6 +232,15 @@ __global__ __launch_bounds__(BS) void kb_waxpby(const BSeg* __restrict__ segs, i
 // ---------------------------------------------------------------------------
 // launching a planned record
 
+template <typename F>
+static void with_rows(int rows, F&& f)
+{
+    if(rows == 2)
+        f(std::integral_constant<int, 2>{});
+    else
+        f(std::integral_constant<int, 1>{});
+}
+
 template <int BS>
 static void launch_bs(const BLaunch& L, hipStream_t s)
 {
@@ -258,50 +268,64 @@ static void launch_bs(const BLaunch& L, hipStream_t s)
         }
         break;
     case BK_FWD_ZERO:
-        with_window(L.win, [&](auto NN) {
-            constexpr int NS = decltype(NN)::value;
-            if(f64)
-                hipLaunchKernelGGL((kb_fwd_zero<NS, double, double, double, BS>), g, b, 0, s, T, n);
-            else if(L.r_f64)
-                hipLaunchKernelGGL((kb_fwd_zero<NS, __half, float, double, BS>), g, b, 0, s, T, n);
-            else
-                hipLaunchKernelGGL((kb_fwd_zero<NS, __half, float, float, BS>), g, b, 0, s, T, n);
+        with_rows(L.rows, [&](auto RR) {
+            constexpr int R = decltype(RR)::value;
+            with_window(L.win, [&](auto NN) {
+                constexpr int NS = decltype(NN)::value;
+                if(f64)
+                    hipLaunchKernelGGL((kb_fwd_zero<NS, double, double, double, R, BS>), g, b, 0, s, T, n);
+                else if(L.r_f64)
+                    hipLaunchKernelGGL((kb_fwd_zero<NS, __half, float, double, R, BS>), g, b, 0, s, T, n);
+                else
+                    hipLaunchKernelGGL((kb_fwd_zero<NS, __half, float, float, R, BS>), g, b, 0, s, T, n);
+            });
         });
         break;
     case BK_BWD_ZERO:
-        with_window(L.win, [&](auto NN) {
-            constexpr int NS = decltype(NN)::value;
-            if(f64)
-                hipLaunchKernelGGL((kb_bwd_zero<NS, double, double, BS>), g, b, 0, s, T, n);
-            else
-                hipLaunchKernelGGL((kb_bwd_zero<NS, __half, float, BS>), g, b, 0, s, T, n);
+        with_rows(L.rows, [&](auto RR) {
+            constexpr int R = decltype(RR)::value;
+            with_window(L.win, [&](auto NN) {
+                constexpr int NS = decltype(NN)::value;
+                if(f64)
+                    hipLaunchKernelGGL((kb_bwd_zero<NS, double, double, R, BS>), g, b, 0, s, T, n);
+                else
+                    hipLaunchKernelGGL((kb_bwd_zero<NS, __half, float, R, BS>), g, b, 0, s, T, n);
+            });
         });
         break;
     case BK_FWD_FULL:
-        if(L.all_u)
-            with_window_pair(L.win, L.win_tail, [&](auto NA_, auto NB_) {
-                constexpr int NA = decltype(NA_)::value, NB = decltype(NB_)::value;
-                if(f64)
-                    hipLaunchKernelGGL((kb_fwd_full_u<NA, NB, double, double, double, BS>), g, b, 0, s, T, n);
-                else if(L.r_f64)
-                    hipLaunchKernelGGL((kb_fwd_full_u<NA, NB, __half, float, double, BS>), g, b, 0, s, T, n);
-                else
-                    hipLaunchKernelGGL((kb_fwd_full_u<NA, NB, __half, float, float, BS>), g, b, 0, s, T, n);
-            });
-        else if(f64)
-            hipLaunchKernelGGL((kb_fwd_full<double, double, double, BS>), g, b, 0, s, T, n);
-        else if(L.r_f64)
-            hipLaunchKernelGGL((kb_fwd_full<__half, float, double, BS>), g, b, 0, s, T, n);
-        else
-            hipLaunchKernelGGL((kb_fwd_full<__half, float, float, BS>), g, b, 0, s, T, n);
+        with_rows(L.rows, [&](auto RR) {
+            constexpr int R = decltype(RR)::value;
+            if(L.all_u)
+                with_window_pair(L.win, L.win_tail, [&](auto NA_, auto NB_) {
+                    constexpr int NA = decltype(NA_)::value, NB = decltype(NB_)::value;
+                    if(f64)
+                        hipLaunchKernelGGL((kb_fwd_full_u<NA, NB, double, double, double, R, BS>), g, b, 0, s, T,
+                                           n);
+                    else if(L.r_f64)
+                        hipLaunchKernelGGL((kb_fwd_full_u<NA, NB, __half, float, double, R, BS>), g, b, 0, s, T,
+                                           n);
+                    else
+                        hipLaunchKernelGGL((kb_fwd_full_u<NA, NB, __half, float, float, R, BS>), g, b, 0, s, T, n);
+                });
+            else if(f64)
+                hipLaunchKernelGGL((kb_fwd_full<double, double, double, R, BS>), g, b, 0, s, T, n);
+            else if(L.r_f64)
+                hipLaunchKernelGGL((kb_fwd_full<__half, float, double, R, BS>), g, b, 0, s, T, n);
+            else
+                hipLaunchKernelGGL((kb_fwd_full<__half, float, float, R, BS>), g, b, 0, s, T, n);
+        });
         break;
     case BK_BWD_FULL:
-        with_window(L.win, [&](auto NN) {
-            constexpr int NS = decltype(NN)::value;
-            if(f64)
-                hipLaunchKernelGGL((kb_bwd_full<NS, double, double, BS>), g, b, 0, s, T, n);
-            else
-                hipLaunchKernelGGL((kb_bwd_full<NS, __half, float, BS>), g, b, 0, s, T, n);
+        with_rows(L.rows, [&](auto RR) {
+            constexpr int R = decltype(RR)::value;
+            with_window(L.win, [&](auto NN) {
+                constexpr int NS = decltype(NN)::value;
+                if(f64)
+                    hipLaunchKernelGGL((kb_bwd_full<NS, double, double, R, BS>), g, b, 0, s, T, n);
+                else
+                    hipLaunchKernelGGL((kb_bwd_full<NS, __half, float, R, BS>), g, b, 0, s, T, n);
+            });
         });
         break;
     case BK_RESTRICT:
@@ -365,6 +389,8 @@ struct Planner
     int next_bf32 = 0;   // the next vector launch reads an f32 z
     // slot windows of the next sweep launch (full rows unless set)
     int next_win = W, next_tail = W, next_all_u = 0;
+    // rows per thread of the next sweep launch (2 needs even segment offsets and sizes)
+    int next_rows = 1;
 
     // widest window of kind `wk` of colour c on level l over the problems
     // that have the colour
@@ -381,13 +407,25 @@ struct Planner
     int add(int kind, int block, bool vector_kind, int r_f64, int dot, double beta, Fill&& fill)
     {
         std::vector<BSeg> segs;
-        int64_t nb = 0;
+        std::vector<size_t> prob;
         for(size_t k = 0; k < B.probs.size(); ++k)
         {
             BSeg sg;
             std::memset(&sg, 0, sizeof(sg));
             sg.gate = &B.probs[k]->sc->done;
             if(!fill((int)k, sg) || sg.size <= 0) continue;
+            segs.push_back(sg);
+            prob.push_back(k);
+        }
+        // two rows per thread load R consecutive rows of a slot as one vector,
+        // which needs even offsets, sizes and ELL strides (as fit() in ops.hip)
+        int rows = next_rows;
+        for(const BSeg& sg : segs)
+            if((sg.off % 2) != 0 || (sg.size % 2) != 0 || (sg.m % 2) != 0) rows = 1;
+        int64_t nb = 0;
+        for(size_t j = 0; j < segs.size(); ++j)
+        {
+            BSeg& sg = segs[j];
             if(vector_kind)
             {
                 int64_t g = ceil_div(sg.size, (int64_t)block * 4);
@@ -395,16 +433,15 @@ struct Planner
             }
             else
             {
-                sg.nblk = ceil_div(sg.size, block);
+                sg.nblk = ceil_div(ceil_div(sg.size, (int64_t)rows), (int64_t)block);
             }
-            if(sg.ra.partial && sg.nblk > B.probs[k]->partial_cap)
+            if(sg.ra.partial && sg.nblk > B.probs[prob[j]]->partial_cap)
             {
                 set_error("batched reduction exceeds the partial capacity");
                 return -2;
             }
             sg.blk0 = nb;
             nb += sg.nblk;
-            segs.push_back(sg);
         }
         BLaunch L;
         std::memset(&L, 0, sizeof(L));
@@ -420,9 +457,11 @@ struct Planner
         L.win = next_win;
         L.win_tail = next_tail;
         L.all_u = next_all_u;
+        L.rows = rows;
         next_bf32 = 0;
         next_win = next_tail = W;
         next_all_u = 0;
+        next_rows = 1;
         if(!segs.empty())
         {
             BSeg* d = nullptr;
@@ -485,6 +524,7 @@ struct Planner
         for(int c = 0; c < C; ++c)
         {
             next_win = widest(l, WIN_LOWER, c);
+            next_rows = B.rows[l];
             HPCGX_TRY(add(BK_FWD_ZERO, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) { return level_seg(k, s, c); }));
         }
         // the pre-smoother keeps its upper sums for the post-smoother where
@@ -492,6 +532,7 @@ struct Planner
         for(int c = C - 2; c >= 0; --c)
         {
             next_win = widest(l, WIN_UPPER, c);
+            next_rows = B.rows[l];
             HPCGX_TRY(add(BK_BWD_ZERO, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
                 if(!(c <= B.probs[k]->L[l].ncolors - 2 && level_seg(k, s, c))) return false;
                 if(!last && B.probs[k]->L[l].u_ok) s.t = vecs(B.probs[k])[l].t;
@@ -535,6 +576,7 @@ struct Planner
                 next_win = widest(l, WIN_LOWER, c);
                 next_tail = widest(l, WIN_TAIL, c);
             }
+            next_rows = B.rows[l];
             HPCGX_TRY(add(BK_FWD_FULL, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
                 if(!level_seg(k, s, c)) return false;
                 const bool before_last = c < B.probs[k]->L[l].ncolors - 1;
@@ -546,6 +588,7 @@ struct Planner
         for(int c = C - 2; c >= 0; --c)
         {
             next_win = widest(l, WIN_UPPER, c);
+            next_rows = B.rows[l];
             HPCGX_TRY(add(BK_BWD_FULL, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
                 if(!(c <= B.probs[k]->L[l].ncolors - 2 && level_seg(k, s, c))) return false;
                 s.t = vecs(B.probs[k])[l].t;
@@ -720,6 +763,19 @@ int batch_set_block(Batch& B, int level, int block)
         return -2;
     }
     B.block[level] = block;
+    HPCGX_HIP(hipStreamSynchronize(B.s));
+    free_plans(B);
+    return 0;
+}
+
+int batch_set_rows(Batch& B, int level, int rows)
+{
+    if(level < 0 || level >= B.nlevels || (rows != 1 && rows != 2))
+    {
+        set_error("invalid level or rows per thread");
+        return -2;
+    }
+    B.rows[level] = rows;
     HPCGX_HIP(hipStreamSynchronize(B.s));
     free_plans(B);
     return 0;

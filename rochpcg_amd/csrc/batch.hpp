@@ -63,6 +63,7 @@ struct BLaunch
     int win;          // sweeps: widest slot window of the launch's segments (Level::win)
     int win_tail;     // post-smoother forward sweep: widest last-colour window
     int all_u;        // post-smoother forward sweep: every segment reuses its upper sums
+    int rows;         // sweeps: rows per thread (1 or 2)
     int nseg;
     int64_t nblocks;
     BSeg* segs;       // device table
@@ -76,6 +77,7 @@ struct Batch
     hipStream_t s = nullptr;
     bool own_stream = false;
     int block[MAX_LEVELS] = {256, 256, 256, 256, 256, 256, 256, 256};
+    int rows[MAX_LEVELS] = {1, 1, 1, 1, 1, 1, 1, 1};   // sweeps: rows per thread, per level
     std::vector<BSeg*> tables;              // device allocations of the plans
     std::vector<BLaunch> pre[2], iter[2];   // preamble and one iteration, per CG mode
     bool planned = false;
@@ -88,6 +90,9 @@ struct Batch
 int batch_create(Batch*& B, int nprob, const int* dims, int nlevels, bool with_f64, bool with_mixed, hipStream_t s);
 void batch_destroy(Batch* B);
 int batch_set_block(Batch& B, int level, int block);
+// rows per thread (1 or 2) of the sweep launches of a level; a launch whose
+// segments are not all even in offset, size and ELL stride runs with 1
+int batch_set_rows(Batch& B, int level, int rows);
 int batch_cg(Batch& B, int mode, int max_iter, double tol, bool use_graph, int* niters, double* normr,
              double* normr0);
 

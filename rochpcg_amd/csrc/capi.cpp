@@ -549,6 +549,16 @@ int hpcgx_batch_set_block(hpcgx_batch* b, int level, int block)
     return batch_set_block(*b->B, level, block);
 }
 
+int hpcgx_batch_set_rows(hpcgx_batch* b, int level, int rows)
+{
+    if(!b || !b->B)
+    {
+        set_error("null batch");
+        return -2;
+    }
+    return batch_set_rows(*b->B, level, rows);
+}
+
 int hpcgx_batch_cg(hpcgx_batch* b, int mode, int max_iter, double tolerance, int use_graph, int* niters,
                    double* normr, double* normr0)
 {

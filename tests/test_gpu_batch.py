@@ -62,4 +62,15 @@ def test_batched_graph_equals_eager(hip):
     # rounding noise (1e-16 relative), where only an absolute bound is meaningful
     diff = np.abs(c["normr"] - a["normr"])
     assert np.all((diff <= 1e-10 * a["normr"]) | (diff <= 1e-15 * a["normr0"]))
+    xc = [P.vec["x"].host() for P in B.problems]
+    # two rows per thread in the sweeps (where every segment allows it) leave
+    # the per-row arithmetic and the reductions as they were: bit for bit
+    for lv in range(4):
+        B.set_rows(lv, 2)
+    for P in B.problems:
+        P.vec["x"].upload(np.zeros(P.m))
+    d = B.cg(max_iter=25, use_graph=True)
+    assert np.array_equal(d["normr"], c["normr"])
+    for P, xv in zip(B.problems, xc):
+        assert np.array_equal(P.vec["x"].host(), xv)
     B.close()
