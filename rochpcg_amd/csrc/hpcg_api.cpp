@@ -51,6 +51,11 @@ Context& ctx()
            hipMalloc(&c.dout, 8 * sizeof(double)) != hipSuccess)
         {
             fprintf(stderr, "rochpcg_amd: cannot initialise the HIP context\n");
+            // release what was made, so that a later call starts clean
+            if(c.dout) (void)hipFree(c.dout);
+            if(c.partial) (void)hipFree(c.partial);
+            if(c.s) (void)hipStreamDestroy(c.s);
+            c = Context();
             return c;
         }
         c.ok = true;
