@@ -411,3 +411,25 @@ def test_fewer_levels_match_oracle(hip, oracle_mod, nlevels):
     assert res["niters"] == 30
     assert_history_close(res["history"], ref["history"])
     G.close()
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 8), (40, 8, 16), (8, 24, 40), (48, 16, 8)])
+def test_small_and_odd_hierarchies_match_oracle(hip, oracle_mod, dims):
+    """Hierarchies down to 1x1x1 and odd coarse grids (5, 3, 1 points per
+    side): colourings other than the parity classes, colours of odd size (one
+    row per thread there) and windows between the compiled buckets.  V-cycle
+    and 20-iteration CG history against the oracle, with two-rows-per-thread
+    shapes requested everywhere."""
+    print(f"[gpu] problem {dims}", flush=True)
+    G = _hip_problem(hip, dims, stores=hip.STORE_F64)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    G.set_shape(128, 2)
+    rng = np.random.default_rng(5)
+    r = rng.uniform(-1, 1, G.m)
+    dr, dz = hip.DeviceArray.from_host(r), hip.DeviceArray(G.m)
+    G.mg(dr, dz)
+    assert _rel(dz.host(), O.mg(r)) <= 1e-12
+    res = G.cg(max_iter=20, tol=0.0, use_graph=True)
+    ref = O.cg(O.rhs_ones(), max_iter=20)
+    assert_history_close(res["history"], ref["history"])
+    G.close()
