@@ -375,12 +375,13 @@ def test_iteration_byte_model(hip):
     G.close()
 
 
-@pytest.mark.parametrize("n", [104, 256, 288])
+@pytest.mark.parametrize("n", [80, 96, 104, 112, 120, 128, 256, 280, 288])
 def test_check_niter_against_reference_tolerance(hip, n):
-    """The reference's acceptance check (test_rochpcg.cpp:88-119, 323-324): the
-    multicolour CG reaches the natural-order CG_ref's 50-iteration residual
-    reduction (tests/golden/ref_tolerance.npz, computed by the reference's
-    own _ref code) within check_niter(n) iterations."""
+    """The reference's acceptance check at the cube sides of its own test
+    (test_rochpcg.cpp:65-77, 88-119, 323-324): the multicolour CG reaches the
+    natural-order CG_ref's 50-iteration residual reduction
+    (tests/golden/ref_tolerance.npz, computed by the reference's own _ref code)
+    within check_niter(n) iterations."""
     import os
     from rochpcg_amd import vv
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_tolerance.npz"))

@@ -301,12 +301,12 @@ def test_oracle_matches_golden_kernels(oracle_mod):
     assert np.array_equal(P.mg(r), g["mg"])
 
 
-def test_oracle_matches_golden_reference_tolerance(oracle_mod):
-    """The 104^3 reference tolerance (50 natural-order CG_ref iterations,
-    written from the reference's own _ref code) is reproduced by the oracle
-    to 1e-10 relative (OpenMP reduction order aside)."""
+@pytest.mark.parametrize("n", [80, 96, 104])
+def test_oracle_matches_golden_reference_tolerance(oracle_mod, n):
+    """The reference tolerances at 80^3, 96^3 and 104^3 (50 natural-order
+    CG_ref iterations, written from the reference's own _ref code) are
+    reproduced by the oracle to 1e-10 relative (OpenMP reduction order aside)."""
     g = _golden("ref_tolerance.npz")
-    n = 104
     tol = float(g["reduction"][list(g["n"]).index(n)])
     P = oracle_mod.Problem(n, nlevels=4, order=oracle_mod.ORDER_NATURAL)
     res = P.cg(P.rhs_ones(), max_iter=50)
