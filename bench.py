@@ -111,7 +111,7 @@ def main():
     ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the solve-to-1e-9 comparison")
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of a hipGraph")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (256 threads, 1 row)")
     ap.add_argument("--seed", type=int, default=1234, help="splitmix64 seed of x* in the solve-to-1e-9 leg")
