@@ -46,19 +46,29 @@ def spmv_bytes(m, vb=8):
 
 
 def cpu_baseline(dims, iters, threads):
+    """CG_ref on the host, natural order, b = A*ones, x0 = 0: the reference's own
+    _ref code (CG_ref.cpp, ComputeMG_ref.cpp, ComputeSYMGS_ref.cpp, ... built
+    into oracle/_ref) when that library is present, else the oracle's
+    restatement of it.  OpenMP where the reference has pragmas; SymGS serial."""
     import oracle
     oracle.set_threads(threads)
     t0 = time.perf_counter()
     P = oracle.Problem(*dims, nlevels=4, order=oracle.ORDER_NATURAL)
-    setup = time.perf_counter() - t0
     b = P.rhs_ones()
+    R = oracle.RefHierarchy(P) if oracle.ref_available() else None
+    setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    res = P.cg(b, max_iter=iters, tol=0.0)
+    res = R.cg(b, max_iter=iters, tol=0.0) if R else P.cg(b, max_iter=iters, tol=0.0)
     el = time.perf_counter() - t0
     flops = oracle.hpcg_flops_per_iter(dims) * res["niters"]
+    what = ("the reference's own CG_ref and _ref kernels (built from its sources into oracle/_ref)"
+            if R else "oracle CG_ref restatement")
+    if R:
+        R.close()
     P.close()
-    return dict(value=round(flops / el / 1e9, 3), unit="GFLOP/s", cores=threads, kind="port",
-                sample=f"oracle CG_ref restatement (natural order, OpenMP except SymGS) at "
+    return dict(value=round(flops / el / 1e9, 3), unit="GFLOP/s", cores=threads,
+                kind="reference" if R else "port",
+                sample=f"{what}, natural order, OpenMP except SymGS, at "
                        f"{dims[0]}x{dims[1]}x{dims[2]}, {res['niters']} iterations, "
                        f"{el:.1f} s timed ({setup:.1f} s setup untimed)",
                 s_per_iter=round(el / max(res["niters"], 1), 3))
