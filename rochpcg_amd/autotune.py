@@ -6,9 +6,10 @@ restriction) and both precisions, time each
 compiled launch shape -- rows per thread {1, 2} x threads per block
 {64, 128, 256} -- with HIP events on the solver stream and keep the fastest.
 Results are cached as JSON keyed by the level's grid and the precision, so a
-later run on the same grids applies them without timing.  The reference has
-no tuner; it hard-codes 1024/256/128-thread blocks (ComputeSPMV.cpp:262,
-ComputeSYMGS.cpp:395, ComputeRestriction.cpp:214).
+later run on the same grids applies them without timing.  `tune_batch` does the
+same for a batch of problems, per level, from short batched solves.  The
+reference has no tuner; it hard-codes 1024/256/128-thread blocks
+(ComputeSPMV.cpp:262, ComputeSYMGS.cpp:395, ComputeRestriction.cpp:214).
 """
 from __future__ import annotations
 
