@@ -85,6 +85,8 @@ int hpcgx_vectors(const hpcgx_problem* p, void** ptrs);
 int hpcgx_make_rhs(hpcgx_problem* p, int kind, uint64_t seed);
 
 /* ---- operators (f64 device vectors, multicolour order) ------------------- */
+/* A NULL vector (of a non-empty length) or result pointer is rejected with
+ * -2 before anything is launched. */
 
 /* ComputeSPMV(A, x, y)               ComputeSPMV.cpp:226-278 */
 int hpcgx_spmv(hpcgx_problem* p, int level, const double* x, double* y);

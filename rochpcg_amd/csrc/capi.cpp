@@ -50,6 +50,20 @@ struct hpcgx_batch
         }                                                                                                              \
     } while(0)
 
+// Null device or host pointers are turned into an error here: a kernel given
+// one would fault the device instead of returning.
+#define CHECK_PTRS(...)                                                                                                \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        const void* ptrs_[] = {__VA_ARGS__};                                                                           \
+        for(const void* q_ : ptrs_)                                                                                    \
+            if(!q_)                                                                                                    \
+            {                                                                                                          \
+                set_error("null pointer argument");                                                                    \
+                return -2;                                                                                             \
+            }                                                                                                          \
+    } while(0)
+
 #define NEED_F64(p)                                                                                                    \
     do                                                                                                                 \
     {                                                                                                                  \
@@ -159,6 +173,7 @@ int hpcgx_spmv(hpcgx_problem* p, int level, const double* x, double* y)
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(x, y);
     const Level& L = p->P->L[level];
     // f64 vectors over the f64 store, or over the exact f16 store when that is all there is
     return launch_spmv_f64x(L, L.ell_val ? PREC_F64 : PREC_MIXED, x, y, nullptr, p->P->s, nullptr);
@@ -168,6 +183,7 @@ int hpcgx_symgs(hpcgx_problem* p, int level, const double* r, double* x)
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(r, x);
     NEED_F64(p);
     return launch_symgs(p->P->L[level], PREC_F64, r, true, x, p->P->v64[level].t, p->P->s, nullptr);
 }
@@ -176,6 +192,7 @@ int hpcgx_symgs_zero(hpcgx_problem* p, int level, const double* r, double* x)
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(r, x);
     NEED_F64(p);
     return launch_symgs_zero(p->P->L[level], PREC_F64, r, true, x, p->P->s, nullptr);
 }
@@ -184,6 +201,7 @@ int hpcgx_mg(hpcgx_problem* p, int level, const double* r, double* x)
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(r, x);
     NEED_F64(p);
     return mg(*p->P, level, PREC_F64, r, true, x, nullptr);
 }
@@ -191,6 +209,7 @@ int hpcgx_mg(hpcgx_problem* p, int level, const double* r, double* x)
 int hpcgx_mg_mixed(hpcgx_problem* p, const double* r, float* x)
 {
     CHECK_P(p);
+    CHECK_PTRS(r, x);
     if(!p->P->has_mixed)
     {
         set_error("operation needs the mixed store (HPCGX_STORE_MIXED)");
@@ -203,6 +222,7 @@ int hpcgx_restriction(hpcgx_problem* p, int level, const double* rf, const doubl
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(rf, xf);
     NEED_F64(p);
     if(level + 1 >= p->P->nlevels)
     {
@@ -218,6 +238,7 @@ int hpcgx_prolongation(hpcgx_problem* p, int level, double* xf, double** xc_out)
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(xf);
     NEED_F64(p);
     if(level + 1 >= p->P->nlevels)
     {
@@ -242,6 +263,8 @@ static RedArgs store_red(Problem* P)
 int hpcgx_dot(hpcgx_problem* p, int64_t n, const double* x, const double* y, double* result)
 {
     CHECK_P(p);
+    CHECK_PTRS(result);
+    if(n > 0) CHECK_PTRS(x, y);
     Problem* P = p->P;
     HPCGX_TRY(launch_dot(n, x, true, y, true, store_red(P), P->s, nullptr));
     HPCGX_HIP(hipMemcpyAsync(P->h_out, P->dout, sizeof(double), hipMemcpyDeviceToHost, P->s));
@@ -253,12 +276,15 @@ int hpcgx_dot(hpcgx_problem* p, int64_t n, const double* x, const double* y, dou
 int hpcgx_waxpby(hpcgx_problem* p, int64_t n, double alpha, const double* x, double beta, const double* y, double* w)
 {
     CHECK_P(p);
+    if(n > 0) CHECK_PTRS(x, y, w);
     return launch_waxpby(n, alpha, x, beta, y, w, p->P->s, nullptr);
 }
 
 int hpcgx_fused_waxpby_dot(hpcgx_problem* p, int64_t n, double alpha, const double* x, double* y, double* result)
 {
     CHECK_P(p);
+    CHECK_PTRS(result);
+    if(n > 0) CHECK_PTRS(x, y);
     Problem* P = p->P;
     HPCGX_TRY(launch_axpy_dot(n, alpha, x, y, store_red(P), P->s, nullptr));
     HPCGX_HIP(hipMemcpyAsync(P->h_out, P->dout, sizeof(double), hipMemcpyDeviceToHost, P->s));
@@ -271,6 +297,7 @@ int hpcgx_copy_diagonal(hpcgx_problem* p, int level, double* diag)
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(diag);
     return launch_get_diag(p->P->L[level], diag, p->P->s);
 }
 
@@ -278,12 +305,14 @@ int hpcgx_replace_diagonal(hpcgx_problem* p, int level, const double* diag)
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(diag);
     return launch_set_diag(p->P->L[level], diag, p->P->s);
 }
 
 int hpcgx_testcg_scale(hpcgx_problem* p, double* diag, double* b)
 {
     CHECK_P(p);
+    CHECK_PTRS(diag, b);
     return launch_testcg_scale(p->P->L[0].m, diag, b, p->P->s);
 }
 
@@ -500,6 +529,7 @@ int hpcgx_memcpy_d2d(void* dst, const void* src, size_t bytes)
 
 int hpcgx_device_count(int* count)
 {
+    CHECK_PTRS(count);
     *count = 0;
     HPCGX_HIP(hipGetDeviceCount(count));
     return 0;

@@ -108,6 +108,25 @@ void* dalloc(size_t bytes)
     return p;
 }
 
+// The reference asserts its vector lengths (ComputeSPMV.cpp:228-229,
+// ComputeSYMGS.cpp:371 and 408, ComputeMG.cpp:66, ComputeWAXPBY.cpp:119-121
+// and 205-206, ComputeDotProduct.cpp:196-197); this layer checks them in every
+// build and returns an error before launching, as it does for a vector
+// without device storage, since a kernel given either would fault the device.
+bool vec_ok(const char* fn, const char* name, const Vector& v, local_int_t need, bool exact)
+{
+    if(v.d_values && (exact ? v.localLength == need : v.localLength >= need)) return true;
+    char msg[160];
+    if(!v.d_values)
+        snprintf(msg, sizeof(msg), "%s: vector %s has no device storage", fn, name);
+    else
+        snprintf(msg, sizeof(msg), "%s: vector %s has length %lld, needs %s%lld", fn, name,
+                 (long long)v.localLength, exact ? "" : "at least ", (long long)need);
+    fprintf(stderr, "%s\n", msg);
+    set_error(msg);
+    return false;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -152,7 +171,11 @@ void HIPInitializeVector(Vector& v, local_int_t localLength)
     v.d_values = (double*)dalloc(sizeof(double) * (size_t)localLength);
 }
 
-void HIPZeroVector(Vector& v) { (void)launch_fill(v.localLength, v.d_values, true, 0.0, ctx().s, nullptr); }
+void HIPZeroVector(Vector& v)
+{
+    if(v.localLength > 0 && !vec_ok("HIPZeroVector", "v", v, v.localLength, true)) return;
+    (void)launch_fill(v.localLength, v.d_values, true, 0.0, ctx().s, nullptr);
+}
 
 void HIPFillRandomVector(Vector& v)
 {
@@ -164,6 +187,10 @@ void HIPFillRandomVector(Vector& v)
 
 void HIPCopyVector(const Vector& v, Vector& w)
 {
+    if(v.localLength > 0 &&
+       (!vec_ok("HIPCopyVector", "v", v, v.localLength, true) ||
+        !vec_ok("HIPCopyVector", "w", w, v.localLength, false)))
+        return;
     (void)launch_copy(v.localLength, v.d_values, true, w.d_values, true, ctx().s, nullptr);
 }
 
@@ -411,6 +438,7 @@ int ComputeSPMV(const SparseMatrix& A, Vector& x, Vector& y)
     Problem* P;
     int l;
     if(!level_of(A, P, l)) return 1;
+    if(!vec_ok("ComputeSPMV", "x", x, A.localNumberOfColumns, false) || !vec_ok("ComputeSPMV", "y", y, A.localNumberOfRows, false)) return 1;
     if(A.mgData && &y == A.mgData->Axf) return launch_spmv_injected(P->L[l], x.d_values, y.d_values, P->s) ? 1 : 0;
     return launch_spmv_f64x(P->L[l], PREC_F64, x.d_values, y.d_values, nullptr, P->s, nullptr) ? 1 : 0;
 }
@@ -420,6 +448,7 @@ int ComputeSYMGS(const SparseMatrix& A, const Vector& r, Vector& x)
     Problem* P;
     int l;
     if(!level_of(A, P, l)) return 1;
+    if(!vec_ok("ComputeSYMGS", "r", r, A.localNumberOfRows, false) || !vec_ok("ComputeSYMGS", "x", x, A.localNumberOfColumns, true)) return 1;
     return launch_symgs(P->L[l], PREC_F64, r.d_values, true, x.d_values, P->v64[l].t, P->s, nullptr) ? 1 : 0;
 }
 
@@ -428,6 +457,9 @@ int ComputeSYMGSZeroGuess(const SparseMatrix& A, const Vector& r, Vector& x)
     Problem* P;
     int l;
     if(!level_of(A, P, l)) return 1;
+    if(!vec_ok("ComputeSYMGSZeroGuess", "r", r, A.localNumberOfRows, false) ||
+       !vec_ok("ComputeSYMGSZeroGuess", "x", x, A.localNumberOfColumns, true))
+        return 1;
     return launch_symgs_zero(P->L[l], PREC_F64, r.d_values, true, x.d_values, P->s, nullptr) ? 1 : 0;
 }
 
@@ -436,6 +468,7 @@ int ComputeMG(const SparseMatrix& A, const Vector& r, Vector& x)
     Problem* P;
     int l;
     if(!level_of(A, P, l)) return 1;
+    if(!vec_ok("ComputeMG", "r", r, A.localNumberOfRows, false) || !vec_ok("ComputeMG", "x", x, A.localNumberOfColumns, true)) return 1;
     return mg(*P, l, PREC_F64, r.d_values, true, x.d_values, nullptr) ? 1 : 0;
 }
 
@@ -444,6 +477,7 @@ int ComputeRestriction(const SparseMatrix& A, const Vector& rf)
     Problem* P;
     int l;
     if(!level_of(A, P, l) || !A.mgData) return 1;
+    if(!vec_ok("ComputeRestriction", "rf", rf, A.localNumberOfRows, false)) return 1;
     return launch_restrict_diff(P->L[l], rf.d_values, A.mgData->Axf->d_values, A.mgData->rc->d_values, P->s) ? 1 : 0;
 }
 
@@ -452,6 +486,9 @@ int ComputeFusedSpMVRestriction(const SparseMatrix& A, const Vector& rf, Vector&
     Problem* P;
     int l;
     if(!level_of(A, P, l) || !A.mgData) return 1;
+    if(!vec_ok("ComputeFusedSpMVRestriction", "rf", rf, A.localNumberOfRows, false) ||
+       !vec_ok("ComputeFusedSpMVRestriction", "xf", xf, A.localNumberOfColumns, false))
+        return 1;
     return launch_restrict(P->L[l], PREC_F64, rf.d_values, true, xf.d_values, A.mgData->rc->d_values, P->s, nullptr)
                ? 1
                : 0;
@@ -462,6 +499,7 @@ int ComputeProlongation(const SparseMatrix& Af, Vector& xf)
     Problem* P;
     int l;
     if(!level_of(Af, P, l) || !Af.mgData) return 1;
+    if(!vec_ok("ComputeProlongation", "xf", xf, Af.localNumberOfRows, false)) return 1;
     return launch_prolong(P->L[l], PREC_F64, Af.mgData->xc->d_values, xf.d_values, P->s, nullptr) ? 1 : 0;
 }
 
@@ -470,6 +508,8 @@ int ComputeDotProduct(const local_int_t n, const Vector& x, const Vector& y, dou
 {
     (void)isOptimized;
     time_allreduce += 0.0;
+    if(n > 0 && (!vec_ok("ComputeDotProduct", "x", x, n, false) || !vec_ok("ComputeDotProduct", "y", y, n, false)))
+        return 1;
     if(launch_dot(n, x.d_values, true, y.d_values, true, store_red(), ctx().s, nullptr)) return 1;
     return fetch_result(result) ? 1 : 0;
 }
@@ -478,6 +518,9 @@ int ComputeWAXPBY(const local_int_t n, const double alpha, const Vector& x, cons
                   Vector& w, bool& isOptimized)
 {
     (void)isOptimized;
+    if(n > 0 && (!vec_ok("ComputeWAXPBY", "x", x, n, false) || !vec_ok("ComputeWAXPBY", "y", y, n, false) ||
+                  !vec_ok("ComputeWAXPBY", "w", w, n, false)))
+        return 1;
     return launch_waxpby(n, alpha, x.d_values, beta, y.d_values, w.d_values, ctx().s, nullptr) ? 1 : 0;
 }
 
@@ -485,6 +528,9 @@ int ComputeFusedWAXPBYDot(const local_int_t n, const double alpha, const Vector&
                           double& time_allreduce)
 {
     time_allreduce += 0.0;
+    if(n > 0 && (!vec_ok("ComputeFusedWAXPBYDot", "x", x, n, false) ||
+                  !vec_ok("ComputeFusedWAXPBYDot", "y", y, n, false)))
+        return 1;
     if(launch_axpy_dot(n, alpha, x.d_values, y.d_values, store_red(), ctx().s, nullptr)) return 1;
     return fetch_result(result) ? 1 : 0;
 }
@@ -509,6 +555,7 @@ int CG(const SparseMatrix& A, CGData& data, const Vector& b, Vector& x, const in
     Problem* P;
     int l;
     if(!level_of(A, P, l) || l != 0) return 1;
+    if(!vec_ok("CG", "b", b, A.localNumberOfRows, false) || !vec_ok("CG", "x", x, A.localNumberOfColumns, true)) return 1;
     (void)data;   // the solver runs on these very buffers (adopted by OptimizeProblem)
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<double> hist(verbose ? (size_t)max_iter + 1 : 0);

@@ -142,7 +142,24 @@ int main(int argc, char** argv)
     printf("testcg_prec_niters %d\n", niters);
     ReplaceMatrixDiagonal(A, d0);
 
-    for(Vector* v : {&Ax, &r0, &x0, &rc1, &s1, &s2, &y1, &z1, &d0, &d1, &bs}) HIPDeleteVector(*v);
+    // vectors the reference asserts against (too short, wrong length, no
+    // device storage) come back as errors before any launch
+    Vector shortv, nullv;
+    HIPInitializeVector(shortv, m - 1);
+    nullv.localLength = m;
+    nullv.d_values = nullptr;
+    nullv.optimizationData = nullptr;
+    double dr = -1.0;
+    const int bad = (ComputeSPMV(A, shortv, Ax) != 0) + (ComputeSPMV(A, x0, shortv) != 0) +
+                    (ComputeSYMGS(A, r0, shortv) != 0) + (ComputeMG(A, nullv, s1) != 0) +
+                    (ComputeDotProduct(m, nullv, x0, dr, t4, opt) != 0) +
+                    (ComputeWAXPBY(m, 1.0, x0, 1.0, shortv, s1, opt) != 0) +
+                    (CG(A, data, shortv, x, 5, 0.0, niters, normr, normr0, times.data(), true, false) != 0);
+    printf("rejected_bad_vectors %d\n", bad);
+    ComputeDotProduct(0, nullv, nullv, dr, t4, opt);
+    printf("empty_dot %.17g\n", dr);
+
+    for(Vector* v : {&Ax, &r0, &x0, &rc1, &s1, &s2, &y1, &z1, &d0, &d1, &bs, &shortv}) HIPDeleteVector(*v);
     DeleteMatrix(A);
     HIPDeleteCGData(data);
     HIPDeleteVector(b);

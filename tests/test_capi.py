@@ -30,6 +30,12 @@ def test_errors_are_reported_not_fatal():
     with pytest.raises(rochpcg_amd.HPCGError, match="invalid"):
         rochpcg_amd.Problem(1, 8, 8, nlevels=1)
     assert "gfx950" in rochpcg_amd.version()
+    # NULL handles and output pointers are refused before any HIP call
+    lib = rochpcg_amd.lib
+    assert lib.hpcgx_device_count(None) == -2
+    assert b"null pointer" in lib.hpcgx_last_error()
+    assert lib.hpcgx_spmv(None, 0, None, None) == -2
+    assert b"null problem" in lib.hpcgx_last_error()
 
 
 def test_hpcg_flop_model():

@@ -43,6 +43,8 @@ def test_drop_in_layer(hip, oracle_mod, dims):
     assert v["cg_tol_scaled"] <= 1e-10 and v["cg_tol_niters"] <= 100
     assert v["solution_error"] < 1e-8
     assert v["testcg_noprec_niters"] <= 12 and v["testcg_prec_niters"] <= 2
+    assert v["rejected_bad_vectors"] == 7         # every ill-sized or storage-less call refused
+    assert v["empty_dot"] == 0.0
     # the same solver through the C ABI gives the same 50-iteration residual
     G = hip.Problem(*dims, nlevels=4, stores=hip.STORE_F64)
     res = G.cg(max_iter=50)

@@ -166,6 +166,13 @@ def test_vector_ops(hip):
     # a length that is not a multiple of any block size
     k = n - 123
     assert abs(G.dot(da, db, n=k) - a[:k] @ db.host()[:k]) <= 1e-12 * abs(a[:k]) @ abs(yb[:k])
+    # an empty dot is zero; NULL vectors are refused on the host before any launch
+    assert G.dot(None, None, n=0) == 0.0
+    for call in (lambda: G.spmv(None, dw), lambda: G.spmv(da, None), lambda: G.dot(da, None),
+                 lambda: G.waxpby(1.0, da, 1.0, None, dw), lambda: G.fused_waxpby_dot(1.0, None, db)):
+        with pytest.raises(hip.HPCGError, match="null pointer"):
+            call()
+    assert np.allclose(db.host(), yb, rtol=1e-15, atol=1e-15)
     G.close()
 
 
