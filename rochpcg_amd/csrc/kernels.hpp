@@ -224,13 +224,18 @@ __device__ __forceinline__ void st_cvt(YT* p, const XT (&v)[R])
 // Row tiles of a whole-level operator: tile t -> (slab j = t / ncolors,
 // colour c = t % ncolors), rows offsets[c] + j*BS*R + [0, BS*R) clipped to the
 // colour.  Consecutive tiles cover the same spatial slab in every colour, and
-// consecutive blocks of an XCD take consecutive runs of `tpb` tiles, so the
-// x gathers of the rows in flight stay inside a compact slab.
+// the blocks in flight on an XCD work on neighbouring tiles, so the x gathers
+// of the rows in flight stay inside a compact slab that the XCD's L2 holds.
+// Blocks are numbered XCD-contiguously (xcd_block).  With tpb tiles per block,
+// consecutive blocks form groups of gx (about what an XCD keeps resident);
+// a group owns gx*tpb consecutive tiles and its blocks step through them side
+// by side (member l takes tiles base + l, + gx, + 2 gx, ...).
 struct Tiles
 {
     int ncolors;
     int jtiles;   // max over colours of ceil(size / (BS*R))
     int tpb;      // tiles per block
+    int gx;       // blocks per group
     int off[MAX_COLORS];
     int size[MAX_COLORS];
 };

@@ -32,12 +32,15 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
                                             YT* __restrict__ y, Tiles tl, RedArgs ra, const int* gate)
 {
     if(gated(gate)) return;
-    const int64_t L = xcd_block(blockIdx.x, gridDim.x);
     const int64_t ntiles = (int64_t)tl.ncolors * tl.jtiles;
-    const int64_t t0 = L * tl.tpb;
-    const int64_t t1 = (t0 + tl.tpb < ntiles) ? t0 + tl.tpb : ntiles;
+    const int64_t L = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t g0 = (L / tl.gx) * tl.gx;   // first block of L's group
+    const int64_t members = (g0 + tl.gx < (int64_t)gridDim.x) ? tl.gx : (int64_t)gridDim.x - g0;
+    const int64_t e = (g0 + members) * tl.tpb;
+    const int64_t t0 = g0 * tl.tpb + (L - g0);
+    const int64_t t1 = e < ntiles ? e : ntiles;
     double acc = 0.0;
-    for(int64_t t = t0; t < t1; ++t)
+    for(int64_t t = t0; t < t1; t += members)
     {
         const int c = (int)(t % tl.ncolors);
         const int64_t i = (t / tl.ncolors) * (BS * R) + (int64_t)threadIdx.x * R;
@@ -366,7 +369,7 @@ int launch_fold(int64_t nparts, const RedArgs& ra, hipStream_t s, const int* gat
     return 0;
 }
 
-// Launch shapes compiled in: rows per thread {1,2,4} x block {64,128,256}.
+// Launch shapes compiled in: rows per thread {1,2} x block {64,128,256}.
 template <typename F>
 static int with_shape(const Shape& sh, F&& f)
 {
@@ -399,7 +402,11 @@ static Shape fit(const Level& L, Shape sh)
     return sh;
 }
 
-static Tiles make_tiles(const Level& L, int rows_per_tile, bool dot, int64_t cap, int64_t& nblocks)
+// With a fused dot, a block walks tpb tiles so that the partials stay few
+// enough for one fold; `resident` (blocks of the kernel the device keeps in
+// flight at once) sets the group size, one XCD's share.
+static Tiles make_tiles(const Level& L, int rows_per_tile, bool dot, int64_t resident, int64_t cap,
+                        int64_t& nblocks)
 {
     Tiles t{};
     t.ncolors = L.ncolors;
@@ -412,12 +419,27 @@ static Tiles make_tiles(const Level& L, int rows_per_tile, bool dot, int64_t cap
     }
     t.jtiles = (int)ceil_div(mx, rows_per_tile);
     const int64_t ntiles = (int64_t)t.ncolors * t.jtiles;
-    // with a fused dot, a block walks tpb consecutive tiles (neighbouring
-    // slabs of the colours) so the partials stay few enough for one fold
     t.tpb = 1;
     while(dot && ceil_div(ntiles, t.tpb) > cap) ++t.tpb;
     nblocks = ceil_div(ntiles, t.tpb);
+    t.gx = (int)std::max<int64_t>(1, resident / NUM_XCD);
     return t;
+}
+
+// Workgroups of `kernel` (BS threads) the device holds at once, queried once
+// per kernel instantiation.
+template <typename K>
+static int64_t resident_blocks(K kernel, int bs, int64_t& cache)
+{
+    if(cache > 0) return cache;
+    int dev = 0, ncu = 0, per_cu = 0;
+    if(hipGetDevice(&dev) == hipSuccess &&
+       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+       hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, bs, 0) == hipSuccess && ncu > 0 && per_cu > 0)
+        cache = (int64_t)ncu * per_cu;
+    else
+        cache = 256 * 2;   // one MI355X at two blocks per CU
+    return cache;
 }
 
 // element-type views of a level's arrays
@@ -442,17 +464,20 @@ static int spmv_t(const Level& L, const void* x, void* y, const RedArgs* ra, hip
     return with_shape(sh, [&](auto RR, auto BB) -> int {
         constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
         int64_t nb;
-        Tiles tl = make_tiles(L, BS * R, ra != nullptr, ra ? std::min<int64_t>(ra->cap, FOLD_MAX) : 0, nb);
         if(ra)
         {
+            const int64_t cap = std::min<int64_t>(ra->cap, FOLD_MAX);
+            static int64_t resident = 0;
+            Tiles tl = make_tiles(L, BS * R, true, resident_blocks(k_spmv<VT, XT, YT, true, R, BS>, BS, resident),
+                                  cap, nb);
             hipLaunchKernelGGL((k_spmv<VT, XT, YT, true, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m, L.ell_col,
                                vals_of<VT>(L), (const XT*)x, (YT*)y, tl, *ra, gate);
             HPCGX_KCHECK();
             return launch_fold(nb, *ra, s, gate);
         }
-        else
-            hipLaunchKernelGGL((k_spmv<VT, XT, YT, false, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m,
-                               L.ell_col, vals_of<VT>(L), (const XT*)x, (YT*)y, tl, RedArgs{}, gate);
+        Tiles tl = make_tiles(L, BS * R, false, 0, 0, nb);
+        hipLaunchKernelGGL((k_spmv<VT, XT, YT, false, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m, L.ell_col,
+                           vals_of<VT>(L), (const XT*)x, (YT*)y, tl, RedArgs{}, gate);
         HPCGX_KCHECK();
         return 0;
     });
