@@ -402,6 +402,29 @@ def test_check_niter_against_reference_tolerance(hip, n):
     G.close()
 
 
+@pytest.mark.parametrize("shape", [(256, 2), (64, 1), (128, 2)])
+def test_fused_spmv_dot_over_many_tile_groups(hip, shape):
+    """The CG's SpMV with its fused p'Ap runs its blocks in groups that step
+    through shared chunks of tiles.  At 80^3 every shape here has 15 full
+    groups and a partial last one.  After one iteration Ap equals the plain
+    SpMV of p bit for bit, and r and its norm match r0 - (r0'z / p'Ap) Ap
+    formed on the host from the same vectors."""
+    G = _hip_problem(hip, (80, 80, 80), stores=hip.STORE_F64)
+    G.set_shape(*shape)
+    res = G.cg(max_iter=1, tol=0.0, use_graph=False)
+    assert res["niters"] == 1
+    b, z, p, Ap, r1 = (G.vec[k].host() for k in ("b", "z", "p", "Ap", "r"))
+    assert np.array_equal(p, z)                     # first iteration: p = z
+    dy = hip.DeviceArray(G.m)
+    G.spmv(G.vec["p"], dy)
+    assert np.array_equal(dy.host(), Ap)
+    alpha = (b @ z) / (p @ Ap)                      # r0 = b (x0 = 0)
+    ref = b - alpha * Ap
+    assert _rel(r1, ref) <= 1e-12
+    assert res["history"][1] == pytest.approx(np.linalg.norm(ref), rel=1e-12)
+    G.close()
+
+
 @pytest.mark.parametrize("nlevels", [1, 2, 3])
 def test_fewer_levels_match_oracle(hip, oracle_mod, nlevels):
     """Hierarchies of 1-3 levels (the coarsest level runs one zero-guess SymGS):
