@@ -104,6 +104,7 @@ int hpcgx_level_info(const hpcgx_problem* p, int level, int64_t* info)
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(info);
     const Level& L = p->P->L[level];
     info[0] = L.nx;
     info[1] = L.ny;
@@ -140,6 +141,7 @@ int hpcgx_level_arrays(const hpcgx_problem* p, int level, void** ptrs)
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(ptrs);
     const Level& L = p->P->L[level];
     void* v[] = {L.ell_col, L.ell_val, L.ell_h, L.diag_idx, L.inv_diag, L.perm, L.colors, L.f2c_nat, L.f2c_fine,
                  L.f2c_coarse, L.desc, L.diag64, L.diag_h};
@@ -150,6 +152,7 @@ int hpcgx_level_arrays(const hpcgx_problem* p, int level, void** ptrs)
 int hpcgx_vectors(const hpcgx_problem* p, void** ptrs)
 {
     CHECK_P(p);
+    CHECK_PTRS(ptrs);
     Problem* P = p->P;
     void* v[] = {P->b, P->x, P->xexact, P->r, P->z64, P->p, P->Ap, P->z32};
     for(int i = 0; i < 8; ++i) ptrs[i] = v[i];
@@ -343,6 +346,7 @@ int hpcgx_cg_iterate(hpcgx_problem* p, int count, int use_graph)
 int hpcgx_cg_iterate_profiled(hpcgx_problem* p, int count, double* phase_ms)
 {
     CHECK_P(p);
+    CHECK_PTRS(phase_ms);
     return cg_iterations_profiled(*p->P, count, phase_ms);
 }
 
@@ -386,6 +390,7 @@ int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps,
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(avg_ms);
     Problem* P = p->P;
     const Level& L = P->L[level];
     const bool mixed = prec != 0;
@@ -475,6 +480,7 @@ int hpcgx_get_shape(const hpcgx_problem* p, int prec, int level, int op, int* bl
 {
     CHECK_P(p);
     CHECK_LEVEL(p, level);
+    CHECK_PTRS(block, rows);
     if(op < 0 || op >= SH_COUNT || prec < 0 || prec > 1)
     {
         set_error("invalid precision or operator");
@@ -494,6 +500,7 @@ int hpcgx_synchronize(hpcgx_problem* p)
 
 int hpcgx_malloc(void** ptr, size_t bytes)
 {
+    CHECK_PTRS(ptr);
     HPCGX_HIP(hipMalloc(ptr, bytes ? bytes : 8));
     return 0;
 }
@@ -506,6 +513,8 @@ int hpcgx_free(void* ptr)
 
 int hpcgx_memcpy_h2d(void* dst, const void* src, size_t bytes)
 {
+    if(bytes == 0) return 0;
+    CHECK_PTRS(dst, src);
     // solver streams are non-blocking: finish their queued work first
     HPCGX_HIP(hipDeviceSynchronize());
     HPCGX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
@@ -514,6 +523,8 @@ int hpcgx_memcpy_h2d(void* dst, const void* src, size_t bytes)
 
 int hpcgx_memcpy_d2h(void* dst, const void* src, size_t bytes)
 {
+    if(bytes == 0) return 0;
+    CHECK_PTRS(dst, src);
     HPCGX_HIP(hipDeviceSynchronize());
     HPCGX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return 0;
@@ -521,6 +532,8 @@ int hpcgx_memcpy_d2h(void* dst, const void* src, size_t bytes)
 
 int hpcgx_memcpy_d2d(void* dst, const void* src, size_t bytes)
 {
+    if(bytes == 0) return 0;
+    CHECK_PTRS(dst, src);
     HPCGX_HIP(hipDeviceSynchronize());
     HPCGX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     HPCGX_HIP(hipDeviceSynchronize());

@@ -36,6 +36,9 @@ def test_errors_are_reported_not_fatal():
     assert b"null pointer" in lib.hpcgx_last_error()
     assert lib.hpcgx_spmv(None, 0, None, None) == -2
     assert b"null problem" in lib.hpcgx_last_error()
+    assert lib.hpcgx_malloc(None, 8) == -2
+    assert lib.hpcgx_memcpy_h2d(None, None, 8) == -2
+    assert lib.hpcgx_memcpy_d2h(None, None, 0) == 0       # nothing to copy
 
 
 def test_hpcg_flop_model():
