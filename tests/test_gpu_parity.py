@@ -402,18 +402,22 @@ def test_check_niter_against_reference_tolerance(hip, n):
     G.close()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("shape", [(256, 2), (64, 1), (128, 2)])
-def test_fused_spmv_dot_over_many_tile_groups(hip, shape):
+def test_fused_spmv_dot_over_many_tile_groups(hip, shape, mixed):
     """The CG's SpMV with its fused p'Ap runs its blocks in groups that step
     through shared chunks of tiles.  At 80^3 every shape here has 15 full
     groups and a partial last one.  After one iteration Ap equals the plain
-    SpMV of p bit for bit, and r and its norm match r0 - (r0'z / p'Ap) Ap
-    formed on the host from the same vectors."""
-    G = _hip_problem(hip, (80, 80, 80), stores=hip.STORE_F64)
+    SpMV of p bit for bit (the f16 values 26 and -1 are exact, so the mixed
+    mode's SpMV over them gives the same sums), and r and its norm match
+    r0 - (r0'z / p'Ap) Ap formed on the host from the same vectors."""
+    G = _hip_problem(hip, (80, 80, 80))
     G.set_shape(*shape)
-    res = G.cg(max_iter=1, tol=0.0, use_graph=False)
+    G.set_shape(*shape, mixed=True)
+    res = G.cg(max_iter=1, tol=0.0, mode=hip.CG_MIXED if mixed else hip.CG_F64, use_graph=False)
     assert res["niters"] == 1
-    b, z, p, Ap, r1 = (G.vec[k].host() for k in ("b", "z", "p", "Ap", "r"))
+    b, p, Ap, r1 = (G.vec[k].host() for k in ("b", "p", "Ap", "r"))
+    z = G.vec["z32"].host().astype(np.float64) if mixed else G.vec["z"].host()
     assert np.array_equal(p, z)                     # first iteration: p = z
     dy = hip.DeviceArray(G.m)
     G.spmv(G.vec["p"], dy)
