@@ -209,6 +209,12 @@ int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps,
  * divisible by `rows` run with 1. */
 int hpcgx_set_shape(hpcgx_problem* p, int prec, int level, int op, int block, int rows);
 int hpcgx_get_shape(const hpcgx_problem* p, int prec, int level, int op, int* block, int* rows);
+/* Block order of the SymGS colour launches (no reference counterpart; a
+ * tuning knob like the launch shape).  columns 0 = choose (planes of the
+ * half-resolution colour grid too large for an XCD's L2 are swept column by
+ * column), 1 = row order, k > 1 = k columns per plane where the colour's tiles
+ * divide so.  Results do not depend on it.  level -1 applies to all. */
+int hpcgx_set_sweep_columns(hpcgx_problem* p, int level, int columns);
 int hpcgx_synchronize(hpcgx_problem* p);
 int hpcgx_malloc(void** ptr, size_t bytes);
 int hpcgx_free(void* ptr);

@@ -76,6 +76,7 @@ def _load():
         "hpcgx_iteration_bytes": ([vp, i32, P(f64), P(f64)], i32),
         "hpcgx_time_operator": ([vp, i32, i32, i32, i32, P(f64)], i32),
         "hpcgx_set_shape": ([vp, i32, i32, i32, i32, i32], i32),
+        "hpcgx_set_sweep_columns": ([vp, i32, i32], i32),
         "hpcgx_get_shape": ([vp, i32, i32, i32, P(i32), P(i32)], i32),
         "hpcgx_synchronize": ([vp], i32),
         "hpcgx_batch_create": ([i32, vp, i32, i32, vp, P(vp)], i32),
@@ -308,6 +309,11 @@ class Problem:
 
     def set_shape(self, block, rows, level=-1, op=-1, mixed=False):
         _check(lib.hpcgx_set_shape(self.h, int(mixed), level, op, block, rows))
+
+    def set_sweep_columns(self, columns, level=-1):
+        """Block order of the SymGS colour launches: 0 chooses, 1 keeps row
+        order, k > 1 sweeps k columns per plane (results do not change)."""
+        _check(lib.hpcgx_set_sweep_columns(self.h, level, columns))
 
     def get_shape(self, level, op, mixed=False):
         b, r = C.c_int(), C.c_int()

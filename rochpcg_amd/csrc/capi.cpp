@@ -476,6 +476,19 @@ int hpcgx_set_shape(hpcgx_problem* p, int prec, int level, int op, int block, in
     return 0;
 }
 
+int hpcgx_set_sweep_columns(hpcgx_problem* p, int level, int columns)
+{
+    CHECK_P(p);
+    if(columns < 0 || level < -1 || level >= p->P->nlevels)
+    {
+        set_error("invalid level or column count");
+        return -2;
+    }
+    for(int l = 0; l < p->P->nlevels; ++l)
+        if(level < 0 || l == level) p->P->L[l].zcols = columns;
+    return 0;
+}
+
 int hpcgx_get_shape(const hpcgx_problem* p, int prec, int level, int op, int* block, int* rows)
 {
     CHECK_P(p);

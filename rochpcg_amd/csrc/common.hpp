@@ -57,6 +57,20 @@ struct Shape
     int rows = 1;      // consecutive rows per thread: 1 or 2
 };
 
+// Block-to-tile order of one colour launch of a sweep.  k <= 1: tile = block.
+// k > 1: the colour's tiles form `planes` planes of k columns of tpc tiles
+// each (in row order); block b takes column b / (planes*tpc), plane
+// (b / tpc) % planes, tile b % tpc of that column -- so the blocks in flight on
+// an XCD march through the planes of one column.  Any such map is a bijection
+// of the colour's tiles, and the rows of a colour are independent, so results
+// do not depend on it.
+struct ZMap
+{
+    int k = 1;
+    int tpc = 0;      // tiles per column and plane
+    int planes = 0;
+};
+
 // Slot windows of the sweeps (see load_window): per colour, the widest window
 // of its rows in the split-padded ELL (the same for every rows-per-thread R,
 // since each window starts at a fixed slot)
@@ -122,6 +136,9 @@ struct Level
     // upper sums over the other colours
     bool u_ok = false;
     unsigned char win[WIN_COUNT][MAX_COLORS] = {};   // [kind][colour]: slots a sweep thread loads per row
+    // columns per plane of the sweeps' block order (ZMap): 0 chooses, 1 keeps
+    // the row order
+    int zcols = 0;
 
     Shape shp[2][SH_COUNT];   // [precision][operator]
 };

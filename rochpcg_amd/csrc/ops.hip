@@ -72,12 +72,19 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
         size = -size;                                                                 \
         Lb_ = (int64_t)gridDim.x - 1 - Lb_;                                           \
     }                                                                                 \
+    if(zm.k > 1)                                                                      \
+    {                                                                                 \
+        const unsigned per_ = (unsigned)zm.planes * (unsigned)zm.tpc;                 \
+        const unsigned lb_ = (unsigned)Lb_, col_ = lb_ / per_, rem_ = lb_ - col_ * per_; \
+        const unsigned pl_ = rem_ / (unsigned)zm.tpc;                                 \
+        Lb_ = ((int64_t)pl_ * zm.k + col_) * zm.tpc + (rem_ - pl_ * (unsigned)zm.tpc); \
+    }                                                                                 \
     const int64_t i = Lb_ * (BS * R) + (int64_t)threadIdx.x * R;                      \
     if(i >= size) return;                                                             \
     const int64_t row = off + i;
 
 template <int NS, typename VT, typename XT, typename RT, int R, int BS>
-__global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col, const VT* __restrict__ val,
+__global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                  const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                  const RT* __restrict__ r, XT* __restrict__ x, const int* gate)
 {
@@ -86,7 +93,7 @@ __global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t
 }
 
 template <int NS, typename VT, typename XT, bool STORE_U, int R, int BS>
-__global__ __launch_bounds__(BS) void k_bwd_zero(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col, const VT* __restrict__ val,
+__global__ __launch_bounds__(BS) void k_bwd_zero(int64_t m, int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                  const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                  XT* __restrict__ x, XT* __restrict__ u, const int* gate)
 {
@@ -95,7 +102,7 @@ __global__ __launch_bounds__(BS) void k_bwd_zero(int64_t m, int64_t off, int64_t
 }
 
 template <typename VT, typename XT, typename RT, bool STORE_T, int R, int BS>
-__global__ __launch_bounds__(BS) void k_fwd_full(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col,
+__global__ __launch_bounds__(BS) void k_fwd_full(int64_t m, int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const RT* __restrict__ r,
                                                  XT* __restrict__ x, XT* __restrict__ t, const int* gate)
 {
@@ -104,7 +111,7 @@ __global__ __launch_bounds__(BS) void k_fwd_full(int64_t m, int64_t off, int64_t
 }
 
 template <int NA, int NB, typename VT, typename XT, typename RT, int R, int BS>
-__global__ __launch_bounds__(BS) void k_fwd_full_u(int64_t m, int64_t off, int64_t size,
+__global__ __launch_bounds__(BS) void k_fwd_full_u(int64_t m, int64_t off, int64_t size, ZMap zm,
                                                    const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                    const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                    const RT* __restrict__ r, XT* __restrict__ x, XT* __restrict__ t,
@@ -115,7 +122,7 @@ __global__ __launch_bounds__(BS) void k_fwd_full_u(int64_t m, int64_t off, int64
 }
 
 template <int NS, typename VT, typename XT, int R, int BS>
-__global__ __launch_bounds__(BS) void k_bwd_full(int64_t m, int64_t off, int64_t size, const int32_t* __restrict__ col, const VT* __restrict__ val,
+__global__ __launch_bounds__(BS) void k_bwd_full(int64_t m, int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                  const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                  const XT* __restrict__ t, XT* __restrict__ x, const int* gate)
 {
@@ -497,6 +504,32 @@ int launch_spmv_f64x(const Level& L, Prec prec, const double* x, double* y, cons
     return spmv_t<__half, double, double>(L, x, y, ra, s, gate);
 }
 
+// The sweeps' block order for a colour of `size` rows in tiles of `tile` rows
+// (grid = its tile count).  Planes of the half-resolution colour grid whose x
+// gathers (7 other colours, 3 planes, 8 B) outgrow an XCD's 4 MB L2 are cut
+// into columns of >= 8192 rows, so that the blocks in flight on an XCD march
+// through one column's planes; Level::zcols > 0 forces a column count.
+static ZMap zmap_for(const Level& L, int64_t size, int tile, unsigned grid)
+{
+    ZMap z;
+    if(L.zcols == 1 || L.ncolors != 8 || ((L.nx | L.ny | L.nz) & 1) != 0) return z;
+    const int64_t plane = (int64_t)(L.nx / 2) * (L.ny / 2), planes = L.nz / 2;
+    if(size != plane * planes || plane % tile) return z;
+    const int64_t tpp = plane / tile;
+    int64_t k = L.zcols;
+    if(k == 0)
+    {
+        if(plane * 8 * 21 <= ((int64_t)6 << 20)) return z;
+        k = 1;
+        while(tpp % (2 * k) == 0 && (tpp / (2 * k)) * tile >= 8192) k *= 2;
+    }
+    if(k <= 1 || tpp % k || (int64_t)grid != planes * tpp) return z;
+    z.k = (int)k;
+    z.tpc = (int)(tpp / k);
+    z.planes = (int)planes;
+    return z;
+}
+
 // one launch per colour c in [c0, c1) (step +1) or (c0, c1] (step -1); a
 // colour whose slot window (kind `wk`, one row per thread) exceeds 16 slots
 // runs with the op's wide shape, the others with its narrow shape
@@ -516,7 +549,7 @@ static int per_color(const Level& L, int prec, int which, int wk, int c_first, i
             // ends on colour 7 reversed, backward starts on colour 6 forwards,
             // so every launch starts on the slab the previous one ended on)
             const int64_t size = (c & 1) ? -(int64_t)L.sizes[c] : (int64_t)L.sizes[c];
-            launch(RR, BB, grid, c, size);
+            launch(RR, BB, grid, c, size, zmap_for(L, L.sizes[c], BS * R, grid.x));
             HPCGX_KCHECK();
             return 0;
         }));
@@ -543,34 +576,34 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
     switch(part)
     {
     case 0:
-        return per_color(L, pr, SH_FWD_ZERO, WIN_LOWER, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
+        return per_color(L, pr, SH_FWD_ZERO, WIN_LOWER, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz, ZMap zm) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             const int n = L.win[WIN_LOWER][c];
             with_window(n, [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
                 hipLaunchKernelGGL((k_fwd_zero<NS, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], sz, L.ell_col, val, dg, L.desc, r, x, gate);
+                                   (int64_t)L.offsets[c], sz, zm, L.ell_col, val, dg, L.desc, r, x, gate);
             });
         });
     case 1:
     case 4:
         if(last < 1) return 0;
-        return per_color(L, pr, SH_BWD_ZERO, WIN_UPPER, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
+        return per_color(L, pr, SH_BWD_ZERO, WIN_UPPER, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz, ZMap zm) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             const int n = L.win[WIN_UPPER][c];
             with_window(n, [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
                 if(part == 4)
                     hipLaunchKernelGGL((k_bwd_zero<NS, VT, XT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                       (int64_t)L.offsets[c], sz, L.ell_col, val, dg, L.desc, x, t, gate);
+                                       (int64_t)L.offsets[c], sz, zm, L.ell_col, val, dg, L.desc, x, t, gate);
                 else
                     hipLaunchKernelGGL((k_bwd_zero<NS, VT, XT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                       (int64_t)L.offsets[c], sz, L.ell_col, val, dg, L.desc, x, t, gate);
+                                       (int64_t)L.offsets[c], sz, zm, L.ell_col, val, dg, L.desc, x, t, gate);
             });
         });
     case 2:
     case 5:
-        return per_color(L, pr, SH_FWD_FULL, part == 5 ? WIN_LOWER : -1, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
+        return per_color(L, pr, SH_FWD_FULL, part == 5 ? WIN_LOWER : -1, 0, last, 1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz, ZMap zm) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             if(c < last && part == 5)
             {
@@ -578,26 +611,26 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
                 with_window_pair(na, nb, [&](auto NA_, auto NB_) {
                     constexpr int NA = decltype(NA_)::value, NB = decltype(NB_)::value;
                     hipLaunchKernelGGL((k_fwd_full_u<NA, NB, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                       (int64_t)L.offsets[c], sz, L.ell_col, val, dg, L.desc, r, x, t,
+                                       (int64_t)L.offsets[c], sz, zm, L.ell_col, val, dg, L.desc, r, x, t,
                                        gate);
                 });
             }
             else if(c < last)
                 hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], sz, L.ell_col, val, r, x, t, gate);
+                                   (int64_t)L.offsets[c], sz, zm, L.ell_col, val, r, x, t, gate);
             else
                 hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], sz, L.ell_col, val, r, x, t, gate);
+                                   (int64_t)L.offsets[c], sz, zm, L.ell_col, val, r, x, t, gate);
         });
     case 3:
         if(last < 1) return 0;
-        return per_color(L, pr, SH_BWD_FULL, WIN_UPPER, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz) {
+        return per_color(L, pr, SH_BWD_FULL, WIN_UPPER, last - 1, 0, -1, [&](auto RR, auto BB, dim3 grid, int c, int64_t sz, ZMap zm) {
             constexpr int R = decltype(RR)::value, BS = decltype(BB)::value;
             const int n = L.win[WIN_UPPER][c];
             with_window(n, [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
                 hipLaunchKernelGGL((k_bwd_full<NS, VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m,
-                                   (int64_t)L.offsets[c], sz, L.ell_col, val, dg, L.desc, (const XT*)t, x, gate);
+                                   (int64_t)L.offsets[c], sz, zm, L.ell_col, val, dg, L.desc, (const XT*)t, x, gate);
             });
         });
     }
