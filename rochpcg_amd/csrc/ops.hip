@@ -507,9 +507,10 @@ int launch_spmv_f64x(const Level& L, Prec prec, const double* x, double* y, cons
 // The sweeps' block order for a colour of `size` rows in tiles of `tile` rows
 // (grid = its tile count).  Planes of the half-resolution colour grid whose x
 // gathers (7 other colours, 3 planes, 8 B) outgrow an XCD's 4 MB L2 are cut
-// into columns of >= 8192 rows, so that the blocks in flight on an XCD march
+// into columns of >= min_rows rows (measured best: 8192 for the f64 sweeps,
+// 4096 for the f16/f32 ones), so that the blocks in flight on an XCD march
 // through one column's planes; Level::zcols > 0 forces a column count.
-static ZMap zmap_for(const Level& L, int64_t size, int tile, unsigned grid)
+static ZMap zmap_for(const Level& L, int64_t size, int tile, unsigned grid, int64_t min_rows)
 {
     ZMap z;
     if(L.zcols == 1 || L.ncolors != 8 || ((L.nx | L.ny | L.nz) & 1) != 0) return z;
@@ -521,7 +522,7 @@ static ZMap zmap_for(const Level& L, int64_t size, int tile, unsigned grid)
     {
         if(plane * 8 * 21 <= ((int64_t)6 << 20)) return z;
         k = 1;
-        while(tpp % (2 * k) == 0 && (tpp / (2 * k)) * tile >= 8192) k *= 2;
+        while(tpp % (2 * k) == 0 && (tpp / (2 * k)) * tile >= min_rows) k *= 2;
     }
     if(k <= 1 || tpp % k || (int64_t)grid != planes * tpp) return z;
     z.k = (int)k;
@@ -549,7 +550,7 @@ static int per_color(const Level& L, int prec, int which, int wk, int c_first, i
             // ends on colour 7 reversed, backward starts on colour 6 forwards,
             // so every launch starts on the slab the previous one ended on)
             const int64_t size = (c & 1) ? -(int64_t)L.sizes[c] : (int64_t)L.sizes[c];
-            launch(RR, BB, grid, c, size, zmap_for(L, L.sizes[c], BS * R, grid.x));
+            launch(RR, BB, grid, c, size, zmap_for(L, L.sizes[c], BS * R, grid.x, prec == PREC_F64 ? 8192 : 4096));
             HPCGX_KCHECK();
             return 0;
         }));
