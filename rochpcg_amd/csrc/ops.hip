@@ -60,10 +60,11 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
     if(DOT) block_partial<BS>(acc, ra.partial, blockIdx.x);
 }
 
-// Per-colour sweeps: block b covers rows off + L*BS*R + [0, BS*R) with
-// L = xcd_block(b).  Consecutive colour launches alternate direction (size < 0
-// marks a reversed launch): each XCD then starts a launch on the slab it
-// finished the previous one on, whose neighbours' x is still in its caches.
+// Per-colour sweeps: block b covers rows off + T*BS*R + [0, BS*R), where T is
+// L = xcd_block(b), mapped through the colour's column order (ZMap) on large
+// planes.  Consecutive colour launches alternate direction (size < 0 marks a
+// reversed launch): each XCD then starts a launch on the slab it finished the
+// previous one on, whose neighbours' x is still in its caches.
 #define SWEEP_ROW()                                                                   \
     if(gated(gate)) return;                                                           \
     int64_t Lb_ = xcd_block(blockIdx.x, gridDim.x);                                   \
