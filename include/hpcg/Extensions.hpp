@@ -15,6 +15,15 @@
 // call the layer starts on the current device at its first use.
 int HPCGX_Init(int device);
 
+// CG_ref (CG_ref.cpp:86-174) on the device, for the reference tolerance of
+// main.cpp:334-344: natural row order, b = the problem's right-hand side
+// (GenerateProblem's A*ones), x = 0, the reference's unoptimized operators
+// with its rounding (ComputeSPMV_ref, ComputeSYMGS_ref as sequential
+// Gauss-Seidel, ComputeMG_ref); only the dot products are summed in another
+// order.  times[0] += wall seconds.
+int HPCGX_ReferenceCG(const SparseMatrix& A, int max_iter, double tolerance, int& niters, double& normr,
+                      double& normr0, double* times);
+
 // TestCG's exaggeration of the diagonal and of b (TestCG.cpp:67-95): the entry
 // at position i is scaled by (g+2)*1e6 when its global row id g < 9 and by 1e6
 // otherwise.  Like the reference, g is the natural row id of position i

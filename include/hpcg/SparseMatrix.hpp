@@ -6,9 +6,14 @@
 #ifndef HPCGX_SPARSEMATRIX_HPP
 #define HPCGX_SPARSEMATRIX_HPP
 
+#include <unordered_map>
+#include <vector>
+
 #include "Geometry.hpp"
 #include "MGData.hpp"
 #include "Vector.hpp"
+
+using GlobalToLocalMap = std::unordered_map<global_int_t, local_int_t>;
 
 struct SparseMatrix_STRUCT
 {
@@ -25,6 +30,11 @@ struct SparseMatrix_STRUCT
     local_int_t** mtxIndL;
     double** matrixValues;
     double** matrixDiagonal;
+    // host row maps of the reference's host setup (GenerateProblem.cpp:500-533);
+    // left empty: on one rank local and global row ids coincide, and this layer
+    // builds no host matrix
+    GlobalToLocalMap globalToLocalMap;
+    std::vector<global_int_t> localToGlobalMap;
     mutable bool isDotProductOptimized;
     mutable bool isSpmvOptimized;
     mutable bool isMgOptimized;
@@ -32,6 +42,22 @@ struct SparseMatrix_STRUCT
     mutable struct SparseMatrix_STRUCT* Ac;
     mutable MGData* mgData;
     void* optimizationData;   // the solver level behind this matrix (set by OptimizeProblem)
+
+    local_int_t* halo_row_ind;   // null: one rank has no halo rows
+
+    // device arrays of the reference's HPCG storage (SparseMatrix.hpp:127-134).
+    // d_localToGlobalMap holds the natural (global) id of every local row, as
+    // GenerateProblem leaves it -- the identity on one rank -- and, as in the
+    // reference, is not permuted by OptimizeProblem (TestCG.cpp:129 reads it).
+    // The others belong to the reference's generate -> colour -> permute
+    // pipeline, which this layer replaces by a single device pass: null.
+    char* d_nonzerosInRow;
+    global_int_t* d_mtxIndG;
+    local_int_t* d_mtxIndL;
+    double* d_matrixValues;
+    local_int_t* d_matrixDiagonal;
+    global_int_t* d_localToGlobalMap;
+    local_int_t* d_rowHash;
 
     // ELL storage (column-major: entry p of row r at p*localNumberOfRows + r,
     // 64-bit offsets), multicolour row order

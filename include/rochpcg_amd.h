@@ -103,12 +103,23 @@ int hpcgx_mg_mixed(hpcgx_problem* p, const double* r, float* x);
  * the next level's rhs buffer (MGData::rc), ComputeRestriction.cpp:199-237;
  * rc_out (may be NULL) receives the device pointer of that buffer */
 int hpcgx_restriction(hpcgx_problem* p, int level, const double* rf, const double* xf, double** rc_out);
+/* The HPCG_REFERENCE build's unfused residual + restriction, as two calls:
+ * ComputeSPMV(A, xf, Axf) with y == MGData::Axf computes A xf at the injected
+ * rows only (ComputeSPMV.cpp:87-128, :238-256; the other rows of y are left
+ * as they are), then ComputeRestriction(A, rf) forms rc = rf - Axf at those
+ * rows into the next level's rhs buffer (ComputeRestriction.cpp:59-79,
+ * :182-197); rc_out (may be NULL) receives that buffer's device pointer */
+int hpcgx_spmv_injected(hpcgx_problem* p, int level, const double* xf, double* Axf);
+int hpcgx_restriction_axf(hpcgx_problem* p, int level, const double* rf, const double* Axf, double** rc_out);
 /* ComputeProlongation(Af, xf): xf += P xc with xc = the next level's
  * solution buffer (MGData::xc), ComputeProlongation.cpp:72-86; xc_out (may be
  * NULL) receives the device pointer of that buffer */
 int hpcgx_prolongation(hpcgx_problem* p, int level, double* xf, double** xc_out);
 /* ComputeDotProduct(n, x, y, result, ...)   ComputeDotProduct.cpp:189-238 */
 int hpcgx_dot(hpcgx_problem* p, int64_t n, const double* x, const double* y, double* result);
+/* ComputeResidual(n, v1, v2, residual): max_i |v1_i - v2_i|
+ * ComputeResidual.cpp:121-152 (synchronises, like the reference) */
+int hpcgx_residual(hpcgx_problem* p, int64_t n, const double* v1, const double* v2, double* result);
 /* ComputeWAXPBY(n, alpha, x, beta, y, w)    ComputeWAXPBY.cpp:111-135 */
 int hpcgx_waxpby(hpcgx_problem* p, int64_t n, double alpha, const double* x, double beta,
                  const double* y, double* w);
@@ -161,6 +172,29 @@ double hpcgx_flops_per_iteration(const hpcgx_problem* p);
  * ReportResults.cpp:107-138) */
 int hpcgx_iteration_bytes(const hpcgx_problem* p, int mode, double* vcycle_bytes, double* iteration_bytes);
 
+/* ---- the reference's natural-order path on the device -------------------- */
+/* Natural row order (GenerateProblem's numbering), the unoptimized operators
+ * of the reference with its operation order and rounding (no fused
+ * multiply-add); the Gauss-Seidel sweeps are the reference's sequential
+ * sweeps, run as wavefronts ix + 2 iy + 4 iz (every lower neighbour of a row
+ * lies in an earlier wavefront).  The benchmark driver takes its reference
+ * tolerance from hpcgx_cg_ref, as main.cpp:334-344 takes it from CG_ref.
+ * All vectors are f64 device vectors in natural order. */
+
+/* ComputeSPMV_ref(A, x, y)           ComputeSPMV_ref.cpp:47-72 */
+int hpcgx_spmv_ref(hpcgx_problem* p, int level, const double* x, double* y);
+/* ComputeSYMGS_ref(A, r, x)          ComputeSYMGS_ref.cpp:54-103 */
+int hpcgx_symgs_ref(hpcgx_problem* p, int level, const double* r, double* x);
+/* ComputeMG_ref(A, r, x)             ComputeMG_ref.cpp:39-63 */
+int hpcgx_mg_ref(hpcgx_problem* p, int level, const double* r, double* x);
+/* CG_ref(A, data, b, x, max_iter, tolerance, niters, normr, normr0, times,
+ *        doPreconditioning)          CG_ref.cpp:86-174
+ * b NULL = the problem's b taken back to natural order; x NULL = a work
+ * vector of the path, zeroed first (else x is the initial guess and receives
+ * the solution).  history/hist_len/times as in hpcgx_cg. */
+int hpcgx_cg_ref(hpcgx_problem* p, const double* b, double* x, int max_iter, double tolerance, int precond,
+                 int* niters, double* normr, double* normr0, double* history, int hist_len, double* times);
+
 /* ---- batched solves (no reference counterpart) --------------------------- */
 
 /* Many independent problems of uneven sizes, each with its own hierarchy
@@ -184,6 +218,9 @@ int hpcgx_batch_set_rows(hpcgx_batch* b, int level, int rows);
  * normr0: arrays of nprob entries (may be NULL) */
 int hpcgx_batch_cg(hpcgx_batch* b, int mode, int max_iter, double tolerance, int use_graph, int* niters,
                    double* normr, double* normr0);
+/* residual history of problem k's last batched CG: normr after iterations
+ * 0..niters into history (at most hist_len entries); *count = entries written */
+int hpcgx_batch_history(hpcgx_batch* b, int k, double* history, int hist_len, int* count);
 
 /* ---- timing and memory helpers ----------------------------------------- */
 

@@ -302,6 +302,81 @@ def hpcg_flops_per_iter(dims, levels=4):
 
 
 # ---------------------------------------------------------------------------
+# numpy restatements for full-size checks (no matrix is stored)
+
+def stencil_apply(x, nx, ny, nz, chunk=32):
+    """y = A x for the 27-point operator of GenerateProblem_ref.cpp:131-182 in
+    natural order (diagonal 26, in-range neighbours -1): y = 27 x - (3x3x3
+    box sum of x, zero outside the grid).  Sums in another order than
+    ComputeSPMV_ref, so it agrees with it to rounding, not bit for bit.
+    Works through z-slabs of `chunk` planes to bound host memory."""
+    X = np.ascontiguousarray(x, np.float64).reshape(nz, ny, nx)
+    y = np.empty_like(X)
+    for z0 in range(0, nz, chunk):
+        z1 = min(nz, z0 + chunk)
+        lo, hi = max(0, z0 - 1), min(nz, z1 + 1)
+        # P[k] holds plane z0 - 1 + k (zero outside the grid)
+        P = np.zeros((z1 - z0 + 2, ny + 2, nx + 2))
+        P[lo - z0 + 1:hi - z0 + 1, 1:-1, 1:-1] = X[lo:hi]
+        S = P[:, :, :-2] + P[:, :, 1:-1] + P[:, :, 2:]
+        S = S[:, :-2] + S[:, 1:-1] + S[:, 2:]
+        S = S[:-2] + S[1:-1] + S[2:]
+        y[z0:z1] = 27.0 * X[z0:z1] - S
+    return y.ravel()
+
+
+def uniform_vector(seed, m):
+    """u_i = orc_uniform(seed, i) for i < m (splitmix64, vectorised)."""
+    i = np.arange(1, m + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def split_ell_rows(nx, ny, nz, perm, iperm, rows):
+    """Expected multicolour ELL entries of multicolour rows `rows` in the
+    split-padded layout: the row's entries (perm of its natural neighbours,
+    ascending, as Permute.cpp:95-149 sorts them) with the lower entries and the
+    diagonal from slot 0 and the upper entries ending at slot 26, -1 / 0.0
+    between.  Returns cols[len(rows), 27] (int32) and vals (float64)."""
+    rows = np.asarray(rows, np.int64)
+    nat = iperm[rows].astype(np.int64)
+    nxy = nx * ny
+    iz, rem = nat // nxy, nat % nxy
+    iy, ix = rem // nx, rem % nx
+    nb = np.full((len(rows), 27), np.iinfo(np.int64).max, np.int64)
+    k = 0
+    for sz in (-1, 0, 1):
+        for sy in (-1, 0, 1):
+            for sx in (-1, 0, 1):
+                z, y, xx = iz + sz, iy + sy, ix + sx
+                ok = (z >= 0) & (z < nz) & (y >= 0) & (y < ny) & (xx >= 0) & (xx < nx)
+                j = np.where(ok, z * nxy + y * nx + xx, 0)
+                nb[:, k] = np.where(ok, perm[j], np.iinfo(np.int64).max)
+                k += 1
+    nb.sort(axis=1)
+    cnt = (nb != np.iinfo(np.int64).max).sum(axis=1)
+    d = (nb < rows[:, None]).sum(axis=1)            # lower entries
+    cols = np.full((len(rows), 27), -1, np.int64)
+    vals = np.zeros((len(rows), 27))
+    slot = np.arange(27)[None, :]
+    lower_or_diag = slot <= d[:, None]
+    cols[lower_or_diag] = nb[lower_or_diag]
+    nup = cnt - d - 1
+    us = 27 - nup
+    upper = slot >= us[:, None]
+    # upper entry q of a row (q = slot - us) is its sorted entry d + 1 + q
+    src = np.clip(d[:, None] + 1 + (slot - us[:, None]), 0, 26)
+    cols[upper] = np.take_along_axis(nb, src, 1)[upper]
+    vals[cols >= 0] = -1.0
+    vals[slot == d[:, None]] = 26.0
+    return cols.astype(np.int32), vals
+
+
+# ---------------------------------------------------------------------------
 _ref = None
 
 

@@ -60,6 +60,9 @@ def _load():
         "hpcgx_mg_mixed": ([vp, vp, vp], i32),
         "hpcgx_restriction": ([vp, i32, vp, vp, P(vp)], i32),
         "hpcgx_prolongation": ([vp, i32, vp, P(vp)], i32),
+        "hpcgx_spmv_injected": ([vp, i32, vp, vp], i32),
+        "hpcgx_restriction_axf": ([vp, i32, vp, vp, P(vp)], i32),
+        "hpcgx_residual": ([vp, i64, vp, vp, P(f64)], i32),
         "hpcgx_dot": ([vp, i64, vp, vp, P(f64)], i32),
         "hpcgx_waxpby": ([vp, i64, f64, vp, f64, vp, vp], i32),
         "hpcgx_fused_waxpby_dot": ([vp, i64, f64, vp, vp, P(f64)], i32),
@@ -72,6 +75,10 @@ def _load():
         "hpcgx_cg_iterate_profiled": ([vp, i32, vp], i32),
         "hpcgx_cg_end": ([vp, P(i32), P(f64), P(f64), vp, i32], i32),
         "hpcgx_set_residual_replacement": ([vp, i32], i32),
+        "hpcgx_spmv_ref": ([vp, i32, vp, vp], i32),
+        "hpcgx_symgs_ref": ([vp, i32, vp, vp], i32),
+        "hpcgx_mg_ref": ([vp, i32, vp, vp], i32),
+        "hpcgx_cg_ref": ([vp, vp, vp, i32, f64, i32, P(i32), P(f64), P(f64), vp, i32, vp], i32),
         "hpcgx_flops_per_iteration": ([vp], f64),
         "hpcgx_iteration_bytes": ([vp, i32, P(f64), P(f64)], i32),
         "hpcgx_time_operator": ([vp, i32, i32, i32, i32, P(f64)], i32),
@@ -85,6 +92,7 @@ def _load():
         "hpcgx_batch_set_block": ([vp, i32, i32], i32),
         "hpcgx_batch_set_rows": ([vp, i32, i32], i32),
         "hpcgx_batch_cg": ([vp, i32, i32, f64, i32, vp, vp, vp], i32),
+        "hpcgx_batch_history": ([vp, i32, vp, i32, P(i32)], i32),
         "hpcgx_malloc": ([P(vp), C.c_size_t], i32),
         "hpcgx_free": ([vp], i32),
         "hpcgx_memcpy_h2d": ([vp, vp, C.c_size_t], i32),
@@ -145,9 +153,15 @@ class DeviceArray:
             raise ValueError(f"size mismatch: {a.size} != {self.n}")
         _check(lib.hpcgx_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes))
 
-    def host(self):
-        out = np.empty(self.n, self.dtype)
-        _check(lib.hpcgx_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
+    def host(self, start=0, count=None):
+        """Elements [start, start + count) copied to a host array."""
+        count = self.n - start if count is None else count
+        if start < 0 or count < 0 or start + count > self.n:
+            raise ValueError(f"range [{start}, {start + count}) outside an array of {self.n}")
+        out = np.empty(count, self.dtype)
+        if count:
+            _check(lib.hpcgx_memcpy_d2h(out.ctypes.data, self.ptr + start * self.dtype.itemsize,
+                                        count * self.dtype.itemsize))
         return out
 
     def free(self):
@@ -253,6 +267,21 @@ class Problem:
             diag_f16=get(ptrs[12], m, np.float16))
         return out
 
+    def level_views(self, level=0):
+        """Device arrays of a level as non-owning DeviceArray views (no copy);
+        the ELL arrays are column-major [27, rows]."""
+        L = self.levels[level]
+        ptrs = (C.c_void_p * 13)()
+        _check(lib.hpcgx_level_arrays(self.h, level, ptrs))
+        m, mc = L.m, L.mc
+        spec = (("ell_col", 27 * m, np.int32), ("ell_val", 27 * m, np.float64),
+                ("ell_f16", 27 * m, np.float16), ("diag_idx", m, np.int32), ("inv_diag", m, np.float64),
+                ("perm", m, np.int32), ("colors", m, np.int32), ("f2c", mc, np.int32),
+                ("f2c_fine", mc, np.int32), ("f2c_coarse", mc, np.int32), ("desc", m, np.int32),
+                ("diag", m, np.float64), ("diag_f16", m, np.float16))
+        return {name: DeviceArray(n, dt, ptr=ptr, owner=False)
+                for (name, n, dt), ptr in zip(spec, ptrs) if ptr and n}
+
     def make_rhs(self, kind=0, seed=0):
         _check(lib.hpcgx_make_rhs(self.h, kind, seed))
 
@@ -276,6 +305,23 @@ class Problem:
         out = C.c_void_p()
         _check(lib.hpcgx_restriction(self.h, level, _p(rf), _p(xf), C.byref(out)))
         return DeviceArray(self.levels[level + 1].m, np.float64, ptr=out.value, owner=False)
+
+    def spmv_injected(self, xf, Axf, level=0):
+        """ComputeSPMV into MGData::Axf: A xf at the injected rows only."""
+        _check(lib.hpcgx_spmv_injected(self.h, level, _p(xf), _p(Axf)))
+
+    def restriction_axf(self, rf, Axf, level=0):
+        """ComputeRestriction (reference mode): rc = rf - Axf at the injected rows."""
+        out = C.c_void_p()
+        _check(lib.hpcgx_restriction_axf(self.h, level, _p(rf), _p(Axf), C.byref(out)))
+        return DeviceArray(self.levels[level + 1].m, np.float64, ptr=out.value, owner=False)
+
+    def residual(self, v1, v2, n=None):
+        """ComputeResidual: max_i |v1_i - v2_i| over the first n entries."""
+        res = C.c_double()
+        n = self.m if n is None else n
+        _check(lib.hpcgx_residual(self.h, n, _p(v1), _p(v2), C.byref(res)))
+        return res.value
 
     def prolongation(self, xf, level=0):
         out = C.c_void_p()
@@ -355,6 +401,32 @@ class Problem:
         return dict(niters=ni.value, normr=nr.value, normr0=nr0.value,
                     history=hist[:min(hist_len, ni.value + 1)])
 
+    # -- the reference's natural-order path on the device ----------------------
+    def spmv_ref(self, x, y, level=0):
+        """ComputeSPMV_ref in natural row order (device vectors)."""
+        _check(lib.hpcgx_spmv_ref(self.h, level, _p(x), _p(y)))
+
+    def symgs_ref(self, r, x, level=0):
+        """ComputeSYMGS_ref: sequential forward + backward Gauss-Seidel in natural order."""
+        _check(lib.hpcgx_symgs_ref(self.h, level, _p(r), _p(x)))
+
+    def mg_ref(self, r, x, level=0):
+        """ComputeMG_ref in natural order."""
+        _check(lib.hpcgx_mg_ref(self.h, level, _p(r), _p(x)))
+
+    def cg_ref(self, b=None, x=None, max_iter=50, tol=0.0, precond=True):
+        """CG_ref in natural order on the device; b None = the problem's b in
+        natural order, x None = a zeroed work vector.  Returns dict(niters,
+        normr, normr0, history, seconds)."""
+        ni, nr, nr0 = C.c_int(), C.c_double(), C.c_double()
+        hist = np.zeros(max_iter + 1)
+        times = np.zeros(8)
+        _check(lib.hpcgx_cg_ref(self.h, _p(b), _p(x), max_iter, tol, int(precond), C.byref(ni),
+                                C.byref(nr), C.byref(nr0), hist.ctypes.data, max_iter + 1,
+                                times.ctypes.data))
+        return dict(niters=ni.value, normr=nr.value, normr0=nr0.value,
+                    history=hist[:ni.value + 1].copy(), seconds=times[0])
+
     def set_residual_replacement(self, every):
         _check(lib.hpcgx_set_residual_replacement(self.h, every))
 
@@ -419,3 +491,10 @@ class Batch:
         _check(lib.hpcgx_batch_cg(self.h, mode, max_iter, tol, int(use_graph), ni.ctypes.data,
                                   nr.ctypes.data, nr0.ctypes.data))
         return dict(niters=ni, normr=nr, normr0=nr0)
+
+    def history(self, k, max_len=4097):
+        """normr after iterations 0..niters of problem k's last batched CG."""
+        hist = np.zeros(max_len)
+        n = C.c_int()
+        _check(lib.hpcgx_batch_history(self.h, k, hist.ctypes.data, max_len, C.byref(n)))
+        return hist[:n.value].copy()

@@ -820,15 +820,23 @@ int batch_cg(Batch& B, int mode, int max_iter, double tol, bool use_graph, int* 
         HPCGX_HIP(hipStreamBeginCapture(B.s, hipStreamCaptureModeThreadLocal));
         const int rc = run_list(B.iter[mode], B.s);
         hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(B.s, &g);
-        if(rc) return rc;
-        if(e != hipSuccess)
+        hipError_t e = hipStreamEndCapture(B.s, &g);
+        if(rc || e != hipSuccess)
         {
+            if(g) (void)hipGraphDestroy(g);
+            if(rc) return rc;
             set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
             return -1;
         }
+        e = hipGraphInstantiate(&B.gexec[mode], g, nullptr, nullptr, 0);
+        if(e != hipSuccess)
+        {
+            (void)hipGraphDestroy(g);
+            B.gexec[mode] = nullptr;
+            set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            return -1;
+        }
         B.graph[mode] = g;
-        HPCGX_HIP(hipGraphInstantiate(&B.gexec[mode], g, nullptr, nullptr, 0));
     }
     const int check = 10;
     std::vector<CGScalars> h(B.probs.size());

@@ -1,12 +1,14 @@
 // capi.cpp -- the extern "C" surface (include/rochpcg_amd.h) over the solver.
 #include "../../include/rochpcg_amd.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <string>
 
 #include "batch.hpp"
 #include "ops.hpp"
+#include "refpath.hpp"
 #include "setup.hpp"
 #include "solver.hpp"
 
@@ -237,6 +239,36 @@ int hpcgx_restriction(hpcgx_problem* p, int level, const double* rf, const doubl
     return launch_restrict(p->P->L[level], PREC_F64, rf, true, xf, rc, p->P->s, nullptr);
 }
 
+int hpcgx_spmv_injected(hpcgx_problem* p, int level, const double* xf, double* Axf)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    CHECK_PTRS(xf, Axf);
+    NEED_F64(p);
+    if(level + 1 >= p->P->nlevels)
+    {
+        set_error("the coarsest level has no injected rows");
+        return -2;
+    }
+    return launch_spmv_injected(p->P->L[level], xf, Axf, p->P->s);
+}
+
+int hpcgx_restriction_axf(hpcgx_problem* p, int level, const double* rf, const double* Axf, double** rc_out)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    CHECK_PTRS(rf, Axf);
+    NEED_F64(p);
+    if(level + 1 >= p->P->nlevels)
+    {
+        set_error("the coarsest level has no restriction");
+        return -2;
+    }
+    double* rc = (double*)p->P->v64[level + 1].r;
+    if(rc_out) *rc_out = rc;
+    return launch_restrict_diff(p->P->L[level], rf, Axf, rc, p->P->s);
+}
+
 int hpcgx_prolongation(hpcgx_problem* p, int level, double* xf, double** xc_out)
 {
     CHECK_P(p);
@@ -270,6 +302,19 @@ int hpcgx_dot(hpcgx_problem* p, int64_t n, const double* x, const double* y, dou
     if(n > 0) CHECK_PTRS(x, y);
     Problem* P = p->P;
     HPCGX_TRY(launch_dot(n, x, true, y, true, store_red(P), P->s, nullptr));
+    HPCGX_HIP(hipMemcpyAsync(P->h_out, P->dout, sizeof(double), hipMemcpyDeviceToHost, P->s));
+    HPCGX_HIP(hipStreamSynchronize(P->s));
+    *result = P->h_out[0];
+    return 0;
+}
+
+int hpcgx_residual(hpcgx_problem* p, int64_t n, const double* v1, const double* v2, double* result)
+{
+    CHECK_P(p);
+    CHECK_PTRS(result);
+    if(n > 0) CHECK_PTRS(v1, v2);
+    Problem* P = p->P;
+    HPCGX_TRY(launch_absdiff_max(n, v1, v2, P->partial, P->partial_cap, P->dout, P->s));
     HPCGX_HIP(hipMemcpyAsync(P->h_out, P->dout, sizeof(double), hipMemcpyDeviceToHost, P->s));
     HPCGX_HIP(hipStreamSynchronize(P->s));
     *result = P->h_out[0];
@@ -354,6 +399,41 @@ int hpcgx_cg_end(hpcgx_problem* p, int* niters, double* normr, double* normr0, d
 {
     CHECK_P(p);
     return cg_end(*p->P, niters, normr, normr0, history, hist_len);
+}
+
+// ---------------------------------------------------------------------------
+int hpcgx_spmv_ref(hpcgx_problem* p, int level, const double* x, double* y)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    CHECK_PTRS(x, y);
+    return ref_spmv(*p->P, level, x, y);
+}
+
+int hpcgx_symgs_ref(hpcgx_problem* p, int level, const double* r, double* x)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    CHECK_PTRS(r, x);
+    return ref_symgs(*p->P, level, r, x);
+}
+
+int hpcgx_mg_ref(hpcgx_problem* p, int level, const double* r, double* x)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    CHECK_PTRS(r, x);
+    return ref_mg(*p->P, level, r, x);
+}
+
+int hpcgx_cg_ref(hpcgx_problem* p, const double* b, double* x, int max_iter, double tolerance, int precond,
+                 int* niters, double* normr, double* normr0, double* history, int hist_len, double* times)
+{
+    CHECK_P(p);
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = ref_cg(*p->P, b, x, max_iter, tolerance, precond != 0, niters, normr, normr0, history, hist_len);
+    if(times) times[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
 }
 
 int hpcgx_set_residual_replacement(hpcgx_problem* p, int every)
@@ -486,6 +566,8 @@ int hpcgx_set_sweep_columns(hpcgx_problem* p, int level, int columns)
     }
     for(int l = 0; l < p->P->nlevels; ++l)
         if(level < 0 || l == level) p->P->L[l].zcols = columns;
+    // a captured CG iteration keeps the block order it was captured with
+    p->P->invalidate_graphs();
     return 0;
 }
 
@@ -624,6 +706,29 @@ int hpcgx_batch_cg(hpcgx_batch* b, int mode, int max_iter, double tolerance, int
         return -2;
     }
     return batch_cg(*b->B, mode, max_iter, tolerance, use_graph != 0, niters, normr, normr0);
+}
+
+int hpcgx_batch_history(hpcgx_batch* b, int k, double* history, int hist_len, int* count)
+{
+    if(!b || !b->B)
+    {
+        set_error("null batch");
+        return -2;
+    }
+    CHECK_PTRS(history, count);
+    if(k < 0 || k >= (int)b->B->probs.size() || hist_len < 1)
+    {
+        set_error("problem index or history length out of range");
+        return -2;
+    }
+    Problem* P = b->B->probs[k];
+    HPCGX_HIP(hipStreamSynchronize(b->B->s));
+    CGScalars h;
+    HPCGX_HIP(hipMemcpy(&h, P->sc, sizeof(h), hipMemcpyDeviceToHost));
+    int n = std::min(h.iter + 1, std::min(hist_len, P->hist_cap + 1));
+    HPCGX_HIP(hipMemcpy(history, P->hist, sizeof(double) * n, hipMemcpyDeviceToHost));
+    *count = n;
+    return 0;
 }
 
 const char* hpcgx_last_error(void) { return last_error(); }

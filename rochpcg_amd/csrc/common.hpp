@@ -223,4 +223,24 @@ __device__ __forceinline__ double block_sum(double v, double* lds)
 // host helpers
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// A device allocation freed when it goes out of scope (every early error
+// return included) unless release() hands it on.
+struct DevTemp
+{
+    void* p = nullptr;
+    DevTemp() = default;
+    DevTemp(const DevTemp&) = delete;
+    DevTemp& operator=(const DevTemp&) = delete;
+    ~DevTemp()
+    {
+        if(p) (void)hipFree(p);
+    }
+    void* release()
+    {
+        void* q = p;
+        p = nullptr;
+        return q;
+    }
+};
+
 }  // namespace hpcgx

@@ -11,16 +11,21 @@
 #include "../../include/hpcg/hpcg.hpp"
 #include "../../include/hpcg/Extensions.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <vector>
 
 #include "ops.hpp"
+#include "refpath.hpp"
 #include "setup.hpp"
 #include "solver.hpp"
 
 using namespace hpcgx;
+
+// the reference's run log (hpcg.hpp:55); unopened unless a driver opens it
+std::ofstream HPCG_fout;
 
 namespace
 {
@@ -241,12 +246,46 @@ void DeleteMGData(MGData& data)
 
 void InitializeSparseMatrix(SparseMatrix& A, Geometry* geom)
 {
-    std::memset(&A, 0, sizeof(A));
+    A.title = nullptr;
     A.geom = geom;
+    A.totalNumberOfRows = 0;
+    A.totalNumberOfNonzeros = 0;
+    A.localNumberOfRows = 0;
+    A.localNumberOfColumns = 0;
+    A.localNumberOfNonzeros = 0;
+    A.numberOfNonzerosPerRow = 0;
+    A.nonzerosInRow = nullptr;
+    A.mtxIndG = nullptr;
+    A.mtxIndL = nullptr;
+    A.matrixValues = nullptr;
+    A.matrixDiagonal = nullptr;
+    A.globalToLocalMap.clear();
+    A.localToGlobalMap.clear();
     A.isDotProductOptimized = true;
     A.isSpmvOptimized = true;
     A.isMgOptimized = true;
     A.isWaxpbyOptimized = true;
+    A.Ac = nullptr;
+    A.mgData = nullptr;
+    A.optimizationData = nullptr;
+    A.halo_row_ind = nullptr;
+    A.d_nonzerosInRow = nullptr;
+    A.d_mtxIndG = nullptr;
+    A.d_mtxIndL = nullptr;
+    A.d_matrixValues = nullptr;
+    A.d_matrixDiagonal = nullptr;
+    A.d_localToGlobalMap = nullptr;
+    A.d_rowHash = nullptr;
+    A.ell_width = 0;
+    A.ell_col_ind = nullptr;
+    A.ell_val = nullptr;
+    A.diag_idx = nullptr;
+    A.inv_diag = nullptr;
+    A.nblocks = 0;
+    A.ublocks = 0;
+    A.sizes = nullptr;
+    A.offsets = nullptr;
+    A.perm = nullptr;
 }
 
 void DeleteMatrix(SparseMatrix& A)
@@ -273,6 +312,10 @@ void DeleteMatrix(SparseMatrix& A)
     delete[] A.sizes;
     delete[] A.offsets;
     A.sizes = A.offsets = nullptr;
+    if(A.d_localToGlobalMap) (void)hipFree(A.d_localToGlobalMap);
+    A.d_localToGlobalMap = nullptr;
+    A.localToGlobalMap.clear();
+    A.globalToLocalMap.clear();
     if(A.geom)
     {
         DeleteGeometry(*A.geom);
@@ -322,6 +365,11 @@ void GenerateProblem(SparseMatrix& A, Vector* b, Vector* x, Vector* xexact)
     if(xexact) HIPInitializeVector(*xexact, m);
     (void)natural_rhs(g.nx, g.ny, g.nz, b ? b->d_values : nullptr, xexact ? xexact->d_values : nullptr,
                       x ? x->d_values : nullptr, ctx().s);
+    // GenerateProblem.cpp:415 and :291: the global id of every local row (the
+    // identity on one rank)
+    if(A.d_localToGlobalMap) (void)hipFree(A.d_localToGlobalMap);
+    A.d_localToGlobalMap = (global_int_t*)dalloc(sizeof(global_int_t) * (size_t)m);
+    if(A.d_localToGlobalMap) (void)iota_i64(m, (int64_t*)A.d_localToGlobalMap, ctx().s);
     // complete on return, like the reference's (GenerateProblem.cpp:462 reads a count back)
     (void)hipStreamSynchronize(ctx().s);
 }
@@ -503,6 +551,16 @@ int ComputeProlongation(const SparseMatrix& Af, Vector& xf)
     return launch_prolong(P->L[l], PREC_F64, Af.mgData->xc->d_values, xf.d_values, P->s, nullptr) ? 1 : 0;
 }
 
+int ComputeResidual(const local_int_t n, const Vector& v1, const Vector& v2, double& residual)
+{
+    if(n > 0 && (!vec_ok("ComputeResidual", "v1", v1, n, false) || !vec_ok("ComputeResidual", "v2", v2, n, false)))
+        return 1;
+    Context& c = ctx();
+    if(!c.ok) return 1;
+    if(launch_absdiff_max(n, v1.d_values, v2.d_values, c.partial, FOLD_MAX, c.dout, c.s)) return 1;
+    return fetch_result(residual) ? 1 : 0;
+}
+
 int ComputeDotProduct(const local_int_t n, const Vector& x, const Vector& y, double& result, double& time_allreduce,
                       bool& isOptimized)
 {
@@ -559,8 +617,41 @@ int CG(const SparseMatrix& A, CGData& data, const Vector& b, Vector& x, const in
     (void)data;   // the solver runs on these very buffers (adopted by OptimizeProblem)
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<double> hist(verbose ? (size_t)max_iter + 1 : 0);
-    const int rc = cg_solve(*P, CG_F64, doPreconditioning, b.d_values, x.d_values, max_iter, tolerance, true, &niters,
-                            &normr, &normr0, verbose ? hist.data() : nullptr, verbose ? max_iter + 1 : 0);
+    int rc = 0;
+    if(!times)
+        rc = cg_solve(*P, CG_F64, doPreconditioning, b.d_values, x.d_values, max_iter, tolerance, true, &niters,
+                      &normr, &normr0, verbose ? hist.data() : nullptr, verbose ? max_iter + 1 : 0);
+    else
+    {
+        // the reference's per-phase timers (CG.cpp:68-82, 187-198): each phase
+        // of every iteration between HIP events on the solver stream, the
+        // iterations launched one kernel at a time; the fused dots are timed
+        // with the kernel they ride on (p'Ap with the SpMV, ||r|| with the
+        // vector update)
+        double ph[PH_COUNT] = {};
+        rc = cg_begin(*P, CG_F64, doPreconditioning, b.d_values, x.d_values, max_iter, tolerance);
+        const int check = 10;
+        while(!rc && P->host_iter < max_iter)
+        {
+            const int n = std::min(check, max_iter - P->host_iter);
+            double part[PH_COUNT] = {};
+            rc = cg_iterations_profiled(*P, n, part);
+            for(int k = 0; k < PH_COUNT; ++k) ph[k] += part[k];
+            if(rc || tolerance <= 0.0) continue;
+            int done = 0;
+            if(hipMemcpyAsync(&done, &P->sc->done, sizeof(int), hipMemcpyDeviceToHost, P->s) != hipSuccess ||
+               hipStreamSynchronize(P->s) != hipSuccess)
+                rc = 1;
+            if(done) break;
+        }
+        if(!rc)
+            rc = cg_end(*P, &niters, &normr, &normr0, verbose ? hist.data() : nullptr, verbose ? max_iter + 1 : 0);
+        const double ms = 1e-3;
+        times[1] += ph[PH_DOT] * ms;
+        times[2] += (ph[PH_WAXPBY] + ph[PH_REPLACE]) * ms;
+        times[3] += ph[PH_SPMV] * ms;
+        times[5] += ph[PH_MG] * ms;
+    }
     if(rc) return 1;
     if(times) times[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if(verbose && A.geom->rank == 0)
@@ -573,6 +664,18 @@ int CG(const SparseMatrix& A, CGData& data, const Vector& b, Vector& x, const in
 
 // ---------------------------------------------------------------------------
 // Extensions.hpp
+
+int HPCGX_ReferenceCG(const SparseMatrix& A, int max_iter, double tolerance, int& niters, double& normr,
+                      double& normr0, double* times)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l) || l != 0) return 1;
+    const auto t0 = std::chrono::steady_clock::now();
+    if(ref_cg(*P, nullptr, nullptr, max_iter, tolerance, true, &niters, &normr, &normr0, nullptr, 0)) return 1;
+    if(times) times[0] += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
 
 int HPCGX_TestCGScale(const SparseMatrix& A, Vector& diagonal, Vector& b)
 {

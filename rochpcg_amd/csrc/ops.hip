@@ -332,6 +332,50 @@ __global__ __launch_bounds__(BS) void k_copy(int64_t n, const T* __restrict__ sr
     if(i < n) dst[i] = (U)src[i];
 }
 
+// max_i |x_i - y_i| (ComputeResidual.cpp:60-119): one partial per block, then
+// one block takes the maximum of the partials (the maximum does not depend on
+// the order, so any grouping gives the same value)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_absdiff_max(int64_t n, const double* __restrict__ x,
+                                                    const double* __restrict__ y, double* __restrict__ partial)
+{
+    int64_t b, e;
+    chunk_range(n, b, e);
+    double v = 0.0;
+    for(int64_t i = b + threadIdx.x; i < e; i += BS) v = fmax(v, fabs(x[i] - y[i]));
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __shared__ double lds[BS / 64];
+    if((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        double mx = 0.0;
+#pragma unroll
+        for(int w = 0; w < BS / 64; ++w) mx = fmax(mx, lds[w]);
+        partial[blockIdx.x] = mx;
+    }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_fold_max(int64_t n, const double* __restrict__ partial, double* __restrict__ out)
+{
+    double v = 0.0;
+    for(int64_t i = threadIdx.x; i < n; i += BS) v = fmax(v, partial[i]);
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __shared__ double lds[BS / 64];
+    if((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        double mx = 0.0;
+#pragma unroll
+        for(int w = 0; w < BS / 64; ++w) mx = fmax(mx, lds[w]);
+        *out = mx;
+    }
+}
+
 // Diagonal access for TestCG (SparseMatrix.cpp:66-152)
 template <typename VT>
 __global__ void k_get_diag(int64_t m, const int32_t* __restrict__ dgi, const VT* __restrict__ val, double* __restrict__ d)
@@ -763,6 +807,19 @@ int launch_dot(int64_t n, const void* x, bool x_f64, const void* y, bool y_f64, 
                            ra, gate);
     HPCGX_KCHECK();
     return launch_fold(g, ra, s, gate);
+}
+
+int launch_absdiff_max(int64_t n, const double* x, const double* y, double* partial, int64_t cap, double* out,
+                       hipStream_t s)
+{
+    constexpr int BS = RED_THREADS;
+    const unsigned g = (unsigned)std::min<int64_t>(red_grid(n), cap);
+    hipLaunchKernelGGL((k_absdiff_max<BS>), dim3(g), dim3(BS), 0, s, n, x, y, partial);
+    HPCGX_KCHECK();
+    hipLaunchKernelGGL((k_fold_max<FOLD_THREADS>), dim3(1), dim3(FOLD_THREADS), 0, s, (int64_t)g,
+                       (const double*)partial, out);
+    HPCGX_KCHECK();
+    return 0;
 }
 
 int launch_waxpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* w, hipStream_t s,

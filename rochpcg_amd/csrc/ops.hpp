@@ -60,6 +60,9 @@ int launch_spmv_injected(const Level& F, const double* x, double* y, hipStream_t
 int launch_restrict_diff(const Level& F, const double* rf, const double* Axf, double* rc, hipStream_t s);
 int launch_dot(int64_t n, const void* x, bool x_f64, const void* y, bool y_f64, const RedArgs& ra, hipStream_t s,
                const int* gate);
+// *out = max_i |x_i - y_i| (two launches; partial holds >= min(blocks, cap) slots)
+int launch_absdiff_max(int64_t n, const double* x, const double* y, double* partial, int64_t cap, double* out,
+                       hipStream_t s);
 int launch_waxpby(int64_t n, double alpha, const double* x, double beta, const double* y, double* w, hipStream_t s,
                   const int* gate);
 int launch_axpy_dot(int64_t n, double alpha, const double* x, double* y, const RedArgs& ra, hipStream_t s,

@@ -365,6 +365,20 @@ int permute_vector(int64_t m, const int32_t* perm, const double* in, double* out
     return 0;
 }
 
+__global__ __launch_bounds__(256) void k_iota_i64(int64_t n, int64_t* __restrict__ p)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if(i < n) p[i] = i;
+}
+
+int iota_i64(int64_t n, int64_t* p, hipStream_t s)
+{
+    if(n <= 0) return 0;
+    hipLaunchKernelGGL(k_iota_i64, dim3(grid_of(n, 256)), dim3(256), 0, s, n, p);
+    HPCGX_HIP(hipGetLastError());
+    return 0;
+}
+
 int natural_rhs(int nx, int ny, int nz, double* b, double* xexact, double* x0, hipStream_t s)
 {
     Grid g{nx, ny, nz};
@@ -379,12 +393,14 @@ int color_level(Level& L, hipStream_t s)
 {
     const int64_t m = L.m;
     Grid g{L.nx, L.ny, L.nz};
-    int32_t *ca = nullptr, *cb = nullptr;
-    unsigned long long* counts = nullptr;
-    HPCGX_HIP(hipMalloc(&ca, sizeof(int32_t) * m));
-    HPCGX_HIP(hipMalloc(&cb, sizeof(int32_t) * m));
-    HPCGX_HIP(hipMalloc(&counts, 2 * sizeof(unsigned long long)));
-    hipLaunchKernelGGL(k_fill_i32, dim3(grid_of(m, 256)), dim3(256), 0, s, m, ca, -1);
+    // device temporaries, freed on every return; the final colours and the
+    // permutations are handed to the level (release) once complete
+    DevTemp ta, tb, tcounts, thist, tbase, tperm, tiperm;
+    HPCGX_HIP(hipMalloc(&ta.p, sizeof(int32_t) * m));
+    HPCGX_HIP(hipMalloc(&tb.p, sizeof(int32_t) * m));
+    HPCGX_HIP(hipMalloc(&tcounts.p, 2 * sizeof(unsigned long long)));
+    unsigned long long* counts = (unsigned long long*)tcounts.p;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_of(m, 256)), dim3(256), 0, s, m, (int32_t*)ta.p, -1);
     // Colour draws as in the reference: srand(0x586744), rand() % 8 for the
     // first eight colours, then counting upwards.
     srand(0x586744);
@@ -396,16 +412,13 @@ int color_level(Level& L, hipStream_t s)
         if(nblocks + 2 > MAX_COLORS)
         {
             set_error("JPL colouring needs more than MAX_COLORS colours");
-            (void)hipFree(ca);
-            (void)hipFree(cb);
-            (void)hipFree(counts);
             return -2;
         }
         const int c1 = (nblocks < 8) ? rand() % 8 : nblocks;
         const int c2 = (nblocks < 8) ? rand() % 8 : nblocks + 1;
         HPCGX_HIP(hipMemsetAsync(counts, 0, 2 * sizeof(unsigned long long), s));
-        hipLaunchKernelGGL(k_jpl_round, dim3(grid_of(m, 256)), dim3(256), 0, s, g, m, ca, cb, c1, c2,
-                           counts);
+        hipLaunchKernelGGL(k_jpl_round, dim3(grid_of(m, 256)), dim3(256), 0, s, g, m, (const int32_t*)ta.p,
+                           (int32_t*)tb.p, c1, c2, counts);
         HPCGX_HIP(hipGetLastError());
         unsigned long long h[2];
         HPCGX_HIP(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -414,25 +427,23 @@ int color_level(Level& L, hipStream_t s)
         L.round_sizes[nblocks + 1] = (int)h[1];
         colored += (int64_t)h[0] + (int64_t)h[1];
         nblocks += 2;
-        std::swap(ca, cb);
+        std::swap(ta.p, tb.p);
         if(++rounds > 4 * MAX_COLORS) break;
     }
-    HPCGX_HIP(hipFree(cb));
-    HPCGX_HIP(hipFree(counts));
-    L.colors = ca;
     if(colored != m)
     {
         set_error("JPL colouring did not colour every row");
         return -2;
     }
+    const int32_t* colors = (const int32_t*)ta.p;
 
     // per-colour counts via the block histogram; ncolors = highest colour + 1
     const int64_t nblk = ceil_div(m, 256);
-    int32_t* hist = nullptr;
-    int64_t* base = nullptr;
-    HPCGX_HIP(hipMalloc(&hist, sizeof(int32_t) * nblk * MAX_COLORS));
-    HPCGX_HIP(hipMalloc(&base, sizeof(int64_t) * nblk * MAX_COLORS));
-    hipLaunchKernelGGL(k_color_hist, dim3((unsigned)nblk), dim3(256), 0, s, m, MAX_COLORS, L.colors, hist, nblk);
+    HPCGX_HIP(hipMalloc(&thist.p, sizeof(int32_t) * nblk * MAX_COLORS));
+    HPCGX_HIP(hipMalloc(&tbase.p, sizeof(int64_t) * nblk * MAX_COLORS));
+    int32_t* hist = (int32_t*)thist.p;
+    int64_t* base = (int64_t*)tbase.p;
+    hipLaunchKernelGGL(k_color_hist, dim3((unsigned)nblk), dim3(256), 0, s, m, MAX_COLORS, colors, hist, nblk);
     HPCGX_HIP(hipGetLastError());
     std::vector<int32_t> hh((size_t)nblk * MAX_COLORS);
     HPCGX_HIP(hipMemcpyAsync(hh.data(), hist, sizeof(int32_t) * hh.size(), hipMemcpyDeviceToHost, s));
@@ -459,16 +470,15 @@ int color_level(Level& L, hipStream_t s)
     }
     L.offsets[ncolors] = (int)run;
     HPCGX_HIP(hipMemcpyAsync(base, hb.data(), sizeof(int64_t) * hb.size(), hipMemcpyHostToDevice, s));
-    int32_t* iperm = nullptr;
-    HPCGX_HIP(hipMalloc(&L.perm, sizeof(int32_t) * m));
-    HPCGX_HIP(hipMalloc(&iperm, sizeof(int32_t) * m));
-    hipLaunchKernelGGL(k_color_scatter, dim3((unsigned)nblk), dim3(256), 0, s, m, ncolors, L.colors, base,
-                       nblk, L.perm, iperm);
+    HPCGX_HIP(hipMalloc(&tperm.p, sizeof(int32_t) * m));
+    HPCGX_HIP(hipMalloc(&tiperm.p, sizeof(int32_t) * m));
+    hipLaunchKernelGGL(k_color_scatter, dim3((unsigned)nblk), dim3(256), 0, s, m, ncolors, colors, base,
+                       nblk, (int32_t*)tperm.p, (int32_t*)tiperm.p);
     HPCGX_HIP(hipGetLastError());
     HPCGX_HIP(hipStreamSynchronize(s));
-    HPCGX_HIP(hipFree(hist));
-    HPCGX_HIP(hipFree(base));
-    L.iperm_tmp = iperm;
+    L.colors = (int32_t*)ta.release();
+    L.perm = (int32_t*)tperm.release();
+    L.iperm_tmp = (int32_t*)tiperm.release();
     return 0;
 }
 

@@ -17,5 +17,7 @@ void free_level(Level& L);
 // natural-order helpers of the drop-in C++ layer
 int natural_transfer(int nxf, int nyf, int nzf, int32_t* f2c, int32_t* c2f, hipStream_t s);
 int permute_vector(int64_t m, const int32_t* perm, const double* in, double* out, hipStream_t s);
+// p[i] = i for i < n
+int iota_i64(int64_t n, int64_t* p, hipStream_t s);
 int natural_rhs(int nx, int ny, int nz, double* b, double* xexact, double* x0, hipStream_t s);
 }  // namespace hpcgx

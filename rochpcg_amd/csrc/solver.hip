@@ -12,6 +12,7 @@
 // per iteration.  A `done` flag written by the ||r|| reduction gates every
 // kernel of later iterations, so iterations can be enqueued (or replayed as a
 // captured hipGraph) ahead of the convergence test.
+#include "refpath.hpp"
 #include "setup.hpp"
 #include "solver.hpp"
 
@@ -125,6 +126,7 @@ void problem_destroy(Problem* P)
 {
     if(!P) return;
     if(P->s) (void)hipStreamSynchronize(P->s);
+    refpath_free(*P);
     for(int g = 0; g < 2; ++g)
     {
         if(P->gexec[g]) (void)hipGraphExecDestroy(P->gexec[g]);
@@ -321,14 +323,22 @@ static int ensure_graph(Problem& P, int g)
     int rc = enqueue_iteration(P, g == 1);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(P.s, &graph);
-    if(rc) return rc;
-    if(e != hipSuccess)
+    if(rc || e != hipSuccess)
     {
+        if(graph) (void)hipGraphDestroy(graph);
+        if(rc) return rc;
         set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
         return -1;
     }
+    e = hipGraphInstantiate(&P.gexec[g], graph, nullptr, nullptr, 0);
+    if(e != hipSuccess)
+    {
+        (void)hipGraphDestroy(graph);
+        P.gexec[g] = nullptr;
+        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        return -1;
+    }
     P.graph[g] = graph;
-    HPCGX_HIP(hipGraphInstantiate(&P.gexec[g], graph, nullptr, nullptr, 0));
     return 0;
 }
 

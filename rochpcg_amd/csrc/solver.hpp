@@ -30,6 +30,8 @@ struct LevelVecs
     void* t = nullptr;   // SymGS partial sums
 };
 
+struct RefPath;
+
 struct Problem
 {
     int nlevels = 0;
@@ -68,6 +70,7 @@ struct Problem
     int graph_mode = -1;
     bool graph_precond = true;
     int graph_replace = -1;
+    RefPath* ref = nullptr;   // natural-order work vectors of the reference path (refpath.hip)
 
     // forget captured graphs (launch shapes or operands changed)
     void invalidate_graphs()

@@ -7,7 +7,9 @@
 #include <cstdlib>
 #include <vector>
 
+#include "Extensions.hpp"
 #include "hpcg.hpp"
+#include "../../include/rochpcg_amd.h"
 
 static double norm(const Vector& v)
 {
@@ -52,8 +54,22 @@ int main(int argc, char** argv)
     Vector Ax;
     HIPInitializeVector(Ax, m);
     ComputeSPMV(A, xexact, Ax);
+    // TestSymmetry.cpp:150-160: ComputeResidual(b, A*xexact), the inf-norm
+    double resid = -1.0;
+    const int rc_resid = ComputeResidual(m, b, Ax, resid);
+    printf("compute_residual %.17g\ncompute_residual_rc %d\n", resid, rc_resid);
     ComputeWAXPBY(m, 1.0, b, -1.0, Ax, Ax, opt);
     printf("spmv_residual %.17g\n", norm(Ax));
+
+    // d_localToGlobalMap: the natural id of every local row (the identity on
+    // one rank), as TestCG.cpp:129 reads it
+    {
+        std::vector<long long> l2g((size_t)m, -1);
+        int ok = A.d_localToGlobalMap != nullptr &&
+                 hpcgx_memcpy_d2h(l2g.data(), A.d_localToGlobalMap, sizeof(long long) * (size_t)m) == 0;
+        for(local_int_t i = 0; ok && i < m; ++i) ok = l2g[(size_t)i] == i;
+        printf("l2g_identity %d\n", ok);
+    }
 
     // reference-mode restriction (SpMV on Axf + ComputeRestriction) equals
     // the fused one (ComputeFusedSpMVRestriction) at the injected points
@@ -118,6 +134,18 @@ int main(int argc, char** argv)
         return 1;
     }
     printf("cg50_niters %d\ncg50_scaled %.17g\n", niters, normr / normr0);
+    // the per-phase timers of CG.cpp:187-198: dot, vector updates, SpMV, MG
+    // (times[4], the all-reduce, stays 0 on one rank)
+    printf("cg_times_phases %d\n", (int)(times[1] > 0.0 && times[2] > 0.0 && times[3] > 0.0 && times[4] == 0.0 &&
+                                          times[5] > 0.0 && times[1] + times[2] + times[3] + times[5] <= times[0]));
+    // CG_ref in natural order on the device (the reference tolerance)
+    {
+        int rn = 0;
+        double rr = 0.0, rr0 = 0.0;
+        std::vector<double> rt(10, 0.0);
+        const int rc_ref = HPCGX_ReferenceCG(A, 50, 0.0, rn, rr, rr0, rt.data());
+        printf("ref_cg_rc %d\nref_cg_niters %d\nref_cg_scaled %.17g\n", rc_ref, rn, rr / rr0);
+    }
     HIPZeroVector(x);
     CG(A, data, b, x, 500, 1e-10, niters, normr, normr0, times.data(), true, false);
     printf("cg_tol_niters %d\ncg_tol_scaled %.17g\n", niters, normr / normr0);

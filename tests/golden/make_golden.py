@@ -86,8 +86,40 @@ def ref_tolerance(sizes=(80, 96, 104, 112, 120, 128, 256, 280, 288)):
     np.savez(os.path.join(HERE, "ref_tolerance.npz"), **out)
 
 
+def history_large(n=320, iters=50, check=5, threads=8):
+    """cg_history_<n>.npz: CG_ref in the JPL multicolour order at n^3 (4 levels,
+    b = A*ones, x0 = 0, `iters` iterations, 8 OpenMP threads): normr per
+    iteration.  The first `check` iterations are recomputed by the reference's
+    own _ref code over the same arrays and must agree to 1e-13 relative (the
+    OpenMP dot reductions of the two builds may combine thread partials in a
+    different order)."""
+    import time
+    oracle.build(ref=True)
+    oracle.set_threads(threads)
+    t0 = time.time()
+    P = oracle.Problem(n, nlevels=4, order=oracle.ORDER_JPL)
+    b = P.rhs_ones()
+    print(f"{n}^3 oracle setup {time.time() - t0:.0f} s", flush=True)
+    t0 = time.time()
+    o = P.cg(b, max_iter=iters)
+    h = o["history"]
+    print(f"{iters} iterations {time.time() - t0:.0f} s; reduction {h[-1] / h[0]:.6e}", flush=True)
+    assert o["niters"] == iters and len(h) == iters + 1
+    R = oracle.RefHierarchy(P)
+    hr = R.history(b, max_iter=check)
+    R.close()
+    rel = np.abs(hr - h[:check + 1]) / h[:check + 1]
+    print(f"reference _ref, iterations 0..{check}: max relative difference {rel.max():.2e}", flush=True)
+    if not np.all(rel <= 1e-13):
+        raise SystemExit(f"oracle and reference disagree on the {n}^3 history: {rel}")
+    np.savez(os.path.join(HERE, f"cg_history_{n}.npz"), n=np.array([n, n, n]), history_jpl=h,
+             threads=np.array(threads), ref_checked=np.array(check))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "ref-tolerance":
         ref_tolerance()
+    elif len(sys.argv) > 1 and sys.argv[1] == "history-large":
+        history_large(int(sys.argv[2]) if len(sys.argv) > 2 else 320)
     else:
         main()

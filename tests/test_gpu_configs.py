@@ -1,0 +1,231 @@
+"""The benchmark configurations at their full sizes on the GPU.
+
+  320^3 FP64       50-iteration residual history against the oracle's CG_ref in
+                   the same colour order (tests/golden/cg_history_320.npz),
+                   1e-12 relative on every iteration.
+  320^3 mixed      x* uniform in [-1,1] (splitmix64 seed 1234), true residual
+                   every 10 iterations, solved to 1e-9 next to the FP64 solve.
+  64 uneven        nx, ny, nz each from {16, 24, ..., 96} (splitmix64 seed 7),
+                   problem k with x* from seed 7 + k, FP64 and mixed to 1e-9.
+  512^3            closed-form colours, permutation and ELL rows whose 64-bit
+                   offsets p*rows + r exceed 2^31; x* from seed 99, FP64 and
+                   mixed (true residual every 10 iterations) to 1e-9.
+
+The residuals and errors of the full-size solves are computed on the host by
+the oracle's numpy restatement of the 27-point operator in natural order
+(oracle.stencil_apply), independently of the GPU's matrix, from the solution
+taken back to natural order through the device's permutation.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _say(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def _natural(vec_mc, perm):
+    """A multicolour vector in natural order: v_nat[i] = v_mc[perm[i]]."""
+    return vec_mc[perm]
+
+
+def _solution_checks(oracle_mod, G, perm, dims):
+    """(true relative residual, relative error) of the problem's x against its
+    b and x*, on the host in natural order."""
+    x = _natural(G.vec["x"].host(), perm)
+    b = _natural(G.vec["b"].host(), perm)
+    xs = _natural(G.vec["xexact"].host(), perm)
+    r = b - oracle_mod.stencil_apply(x, *dims)
+    return float(np.linalg.norm(r) / np.linalg.norm(b)), float(np.linalg.norm(x - xs) / np.linalg.norm(xs))
+
+
+def test_config_fp64_320_history(hip):
+    """FP64 CG at 320^3, b = A*ones, x0 = 0, 50 iterations: every iteration's
+    residual norm within 1e-12 relative of the oracle's CG_ref in the same
+    colour order.  The 50 iterations reduce the residual by only ~4e-3, so
+    the relative bar holds without any allowance for rounding noise."""
+    g = np.load(os.path.join(GOLDEN, "cg_history_320.npz"))
+    hr = g["history_jpl"]
+    _say("config fp64 320^3: 50 iterations against the oracle history")
+    G = hip.Problem(320, nlevels=4, stores=hip.STORE_F64)
+    res = G.cg(max_iter=50, tol=0.0, use_graph=True)
+    G.close()
+    h = res["history"]
+    assert res["niters"] == 50 and len(h) == 51
+    rel = np.abs(h - hr) / hr
+    _say(f"320^3 history: max rel {rel.max():.2e} (iteration {int(rel.argmax())}), "
+         f"reduction {h[-1] / h[0]:.4e}")
+    assert np.all(rel <= 1e-12), rel
+
+
+def test_config_mixed_320(hip, oracle_mod):
+    """Mixed precision at 320^3 with the f64 true residual every 10
+    iterations, against the FP64 solve of the same system."""
+    dims = (320, 320, 320)
+    G = hip.Problem(*dims, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
+    perm = G.level_views(0)["perm"].host()
+    G.make_rhs(1, 1234)
+    xs = oracle_mod.uniform_vector(1234, G.m) * 2.0 - 1.0
+    assert np.array_equal(_natural(G.vec["xexact"].host(), perm), xs)
+    _say("config mixed 320^3: FP64 solve to 1e-9")
+    f64 = G.cg(max_iter=500, tol=1e-9, mode=hip.CG_F64, history=False)
+    r64, e64 = _solution_checks(oracle_mod, G, perm, dims)
+    _say(f"FP64: {f64['niters']} iterations, true residual {r64:.3e}, error {e64:.3e}")
+    G.vec["x"].upload(np.zeros(G.m))
+    G.set_residual_replacement(10)
+    _say("config mixed 320^3: mixed solve to 1e-9, true residual every 10 iterations")
+    mix = G.cg(max_iter=500, tol=1e-9, mode=hip.CG_MIXED, history=False)
+    rmx, emx = _solution_checks(oracle_mod, G, perm, dims)
+    _say(f"mixed: {mix['niters']} iterations, true residual {rmx:.3e}, error {emx:.3e}")
+    G.close()
+    assert f64["normr"] / f64["normr0"] <= 1e-9 and mix["normr"] / mix["normr0"] <= 1e-9
+    assert r64 <= 1.05e-9 and rmx <= 1.05e-9
+    assert emx <= 2.0 * e64
+    assert abs(mix["niters"] - f64["niters"]) <= 3
+
+
+@pytest.fixture(scope="module")
+def batch64(hip):
+    from rochpcg_amd import vv
+    dims = vv.batched_dims(64, seed=7)
+    _say(f"config batched: 64 problems, {sum(a * b * c for a, b, c in dims)} rows")
+    B = hip.Batch(dims, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
+    for k, P in enumerate(B.problems):
+        P.make_rhs(1, 7 + k)
+    yield dims, B
+    B.close()
+
+
+def _batch_solve(hip, oracle_mod, B, dims, mode):
+    for P in B.problems:
+        P.vec["x"].upload(np.zeros(P.m))
+    res = B.cg(max_iter=500, tol=1e-9, mode=mode)
+    out = []
+    for k, (P, d) in enumerate(zip(B.problems, dims)):
+        perm = P.level_views(0)["perm"].host()
+        rt, err = _solution_checks(oracle_mod, P, perm, tuple(int(v) for v in d))
+        out.append((int(res["niters"][k]), res["normr"][k] / res["normr0"][k], rt, err))
+    return out
+
+
+def test_config_batched_64(hip, oracle_mod, batch64):
+    """The 64-problem workload, FP64 and mixed: each problem reaches 1e-9
+    with its true residual (host restatement) at 1e-9 too, the mixed solve's
+    error within 2x of the FP64 one's and its iteration count within 3."""
+    dims, B = batch64
+    for k, (P, d) in enumerate(zip(B.problems, dims)):
+        if k % 16 == 0:   # the seeded x* of a sample of problems
+            xs = oracle_mod.uniform_vector(7 + k, P.m) * 2.0 - 1.0
+            perm = P.level_views(0)["perm"].host()
+            assert np.array_equal(_natural(P.vec["xexact"].host(), perm), xs)
+    _say("batched FP64 solve")
+    f64 = _batch_solve(hip, oracle_mod, B, dims, hip.CG_F64)
+    _say("batched mixed solve")
+    mix = _batch_solve(hip, oracle_mod, B, dims, hip.CG_MIXED)
+    worst = max(range(len(dims)), key=lambda k: f64[k][3])
+    _say(f"FP64: iterations {min(v[0] for v in f64)}-{max(v[0] for v in f64)}, worst true residual "
+         f"{max(v[2] for v in f64):.3e}, worst error {f64[worst][3]:.3e} ({tuple(dims[worst])})")
+    _say(f"mixed: iterations {min(v[0] for v in mix)}-{max(v[0] for v in mix)}, worst true residual "
+         f"{max(v[2] for v in mix):.3e}, worst error ratio "
+         f"{max(m[3] / f[3] for m, f in zip(mix, f64)):.3f}")
+    for k, (f, m) in enumerate(zip(f64, mix)):
+        assert f[1] <= 1e-9 and m[1] <= 1e-9, (k, f, m)
+        assert f[2] <= 1.05e-9 and m[2] <= 1.05e-9, (k, f, m)
+        assert m[3] <= 2.0 * f[3], (k, f, m)
+        assert abs(m[0] - f[0]) <= 3, (k, f, m)
+
+
+def test_config_batched_64_history_vs_oracle(hip, oracle_mod, batch64):
+    """The three smallest problems of the 64-problem batch, FP64: the batched
+    residual history against the oracle's CG_ref in the same colour order.
+    The solves run to 1e-9, so the later iterations sit near rounding noise:
+    per iteration the bar is 1e-12 relative, or 1e-14 of the initial residual
+    (the oracle's own CG_ref on 1 and on 8 threads differs by that much)."""
+    dims, B = batch64
+    for P in B.problems:
+        P.vec["x"].upload(np.zeros(P.m))
+    res = B.cg(max_iter=500, tol=1e-9, mode=hip.CG_F64)
+    rows = [int(a) * int(b) * int(c) for a, b, c in dims]
+    for k in sorted(range(len(dims)), key=lambda j: rows[j])[:3]:
+        d = tuple(int(v) for v in dims[k])
+        O = oracle_mod.Problem(*d, nlevels=4, order=oracle_mod.ORDER_JPL)
+        _, b = O.rhs_random(7 + k)
+        ref = O.cg(b, max_iter=500, tol=1e-9)
+        O.close()
+        h = B.history(k)
+        hr = ref["history"]
+        n = min(len(h), len(hr))
+        rel = np.abs(h[:n] - hr[:n]) / hr[:n]
+        absd = np.abs(h[:n] - hr[:n]) / hr[0]
+        _say(f"problem {k} {d}: {res['niters'][k]} iterations (oracle {ref['niters']}), max rel "
+             f"{rel.max():.2e}, max |dh|/h0 {absd.max():.2e}")
+        assert abs(int(res["niters"][k]) - ref["niters"]) <= 1
+        assert np.all((rel <= 1e-12) | (absd <= 1e-14)), (k, rel, absd)
+
+
+def test_config_512(hip, oracle_mod):
+    """512^3 (134M rows, 27 * rows > 2^31): the level-0 colours and
+    permutation against the closed form, the ELL rows of windows in every
+    colour (every slot p >= 16 sits at p * rows + r >= 2^31) against the
+    oracle's split layout, then the FP64 and the mixed solve (seed 99, true
+    residual every 10 iterations) to 1e-9."""
+    n = 512
+    dims = (n, n, n)
+    _say("config 512^3: setup")
+    G = hip.Problem(*dims, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
+    m = G.m
+    assert 16 * m >= 2 ** 31
+    V = G.level_views(0)
+    colors = V["colors"].host()
+    assert np.array_equal(colors, oracle_mod.closed_form_colors(*dims))
+    perm_cf, sizes, offsets = oracle_mod.perm_from_colors(colors, 8)
+    del colors
+    perm = V["perm"].host()
+    assert np.array_equal(perm, perm_cf)
+    del perm_cf
+    s, o, _ = G.color_table(0)
+    assert np.array_equal(s, sizes) and np.array_equal(o, offsets)
+    iperm = np.empty_like(perm)
+    iperm[perm] = np.arange(m, dtype=np.int32)
+    _say("config 512^3: ELL rows at 64-bit offsets")
+    win = 1024
+    starts = [int(c * (m // 8) + (c * 7919) % (m // 8 - win)) for c in range(8)] + [m - win]
+    for r0 in starts:
+        rows = np.arange(r0, r0 + win)
+        cols_e, vals_e = oracle_mod.split_ell_rows(*dims, perm, iperm, rows)
+        for p in range(27):
+            cols = V["ell_col"].host(p * m + r0, win)
+            assert np.array_equal(cols, cols_e[:, p]), (r0, p)
+            vals = V["ell_val"].host(p * m + r0, win)
+            assert np.array_equal(vals, vals_e[:, p]), (r0, p)
+            if p >= 16:
+                assert np.array_equal(V["ell_f16"].host(p * m + r0, win).astype(np.float64), vals_e[:, p])
+        lower = ((cols_e >= 0) & (cols_e < rows[:, None])).sum(axis=1)
+        assert np.array_equal(V["diag_idx"].host(r0, win), lower)
+        assert np.array_equal(V["desc"].host(r0, win) & 255, lower)
+    del iperm
+    G.make_rhs(1, 99)
+    xs = oracle_mod.uniform_vector(99, m) * 2.0 - 1.0
+    assert np.array_equal(_natural(G.vec["xexact"].host(), perm), xs)
+    del xs
+    _say("config 512^3: FP64 solve to 1e-9")
+    f64 = G.cg(max_iter=600, tol=1e-9, mode=hip.CG_F64, history=False)
+    r64, e64 = _solution_checks(oracle_mod, G, perm, dims)
+    _say(f"FP64: {f64['niters']} iterations, true residual {r64:.3e}, error {e64:.3e}")
+    G.vec["x"].upload(np.zeros(m))
+    G.set_residual_replacement(10)
+    _say("config 512^3: mixed solve to 1e-9, true residual every 10 iterations")
+    mix = G.cg(max_iter=600, tol=1e-9, mode=hip.CG_MIXED, history=False)
+    rmx, emx = _solution_checks(oracle_mod, G, perm, dims)
+    _say(f"mixed: {mix['niters']} iterations, true residual {rmx:.3e}, error {emx:.3e}")
+    G.close()
+    assert f64["normr"] / f64["normr0"] <= 1e-9 and mix["normr"] / mix["normr0"] <= 1e-9
+    assert r64 <= 1.05e-9 and rmx <= 1.05e-9
+    assert emx <= 2.0 * e64
+    assert abs(mix["niters"] - f64["niters"]) <= 3

@@ -34,6 +34,10 @@ def test_drop_in_layer(hip, oracle_mod, dims):
     assert v["rows"] == m and v["nonzeros"] == np.prod([3 * d - 2 for d in dims])
     assert v["nblocks"] == 8 and v["ublocks"] == 6
     assert v["spmv_residual"] == 0.0            # integers in f64: A*ones == b exactly
+    assert v["compute_residual_rc"] == 0 and v["compute_residual"] == 0.0
+    assert v["l2g_identity"] == 1
+    assert v["cg_times_phases"] == 1
+    assert v["ref_cg_rc"] == 0 and v["ref_cg_niters"] == 50
     assert v["restriction_diff"] <= 1e-15
     assert v["symgs_zero_diff"] <= 1e-14
     assert v["prolongation_norm2"] == pytest.approx(v["xc_norm2"], rel=1e-14)
@@ -54,6 +58,11 @@ def test_drop_in_layer(hip, oracle_mod, dims):
     O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
     ref = O.cg(O.rhs_ones(), max_iter=50)
     assert abs(v["cg50_scaled"] - ref["normr"] / ref["normr0"]) <= 1e-15
+    # the reference tolerance from the device CG_ref: the oracle's natural-order
+    # CG_ref to rounding noise
+    N = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_NATURAL)
+    nat = N.cg(N.rhs_ones(), max_iter=50)
+    assert abs(v["ref_cg_scaled"] - nat["normr"] / nat["normr0"]) <= 1e-15
 
 
 @pytest.mark.gpu

@@ -148,6 +148,38 @@ def test_operators_match_oracle(hip, oracle_mod, dims):
     G.close()
 
 
+@pytest.mark.parametrize("dims", [(32, 32, 32), (24, 16, 32)])
+def test_unfused_restriction_matches_oracle(hip, oracle_mod, dims):
+    """The HPCG_REFERENCE-mode residual + restriction (ComputeSPMV into
+    MGData::Axf at the injected rows, then ComputeRestriction) against the
+    oracle's ComputeMG_ref, which runs the full SpMV into Axf and
+    orc_restriction after its pre-smoother: from the same pre-smoothed x,
+    Axf at the injected rows and rc within 1e-12."""
+    G = _hip_problem(hip, dims, stores=hip.STORE_F64)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    rng = np.random.default_rng(19)
+    r = rng.uniform(-1, 1, G.m)
+    O.mg(r)                                   # pre-smoother, Axf, orc_restriction, ...
+    Axf_o, rc_o, _ = O.level_vectors(0)
+    f2c = O.export(0)["f2c"]
+    dr, dx = hip.DeviceArray.from_host(r), hip.DeviceArray(G.m)
+    G.symgs_zero(dr, dx)                      # the same pre-smoother
+    sentinel = np.full(G.m, 7.0)
+    dAxf = hip.DeviceArray.from_host(sentinel)
+    G.spmv_injected(dx, dAxf)
+    Axf = dAxf.host()
+    rows = np.zeros(G.m, bool)
+    rows[f2c] = True
+    assert np.all(Axf[~rows] == 7.0)          # only the injected rows are computed
+    assert _rel(Axf[f2c], Axf_o[f2c]) <= 1e-12
+    rc = G.restriction_axf(dr, dAxf).host()
+    assert _rel(rc, rc_o) <= 1e-12
+    # the fused form gives the same rc from the same x
+    rcf = G.restriction(dr, dx).host()
+    assert _rel(rcf, rc_o) <= 1e-12
+    G.close()
+
+
 def test_vector_ops(hip):
     G = _hip_problem(hip, (32, 32, 32), stores=hip.STORE_F64)
     rng = np.random.default_rng(3)
@@ -159,6 +191,9 @@ def test_vector_ops(hip):
     for alpha, beta in ((1.0, -0.5), (0.25, 1.0), (0.3, 0.7)):
         G.waxpby(alpha, da, beta, db, dw)
         assert np.allclose(dw.host(), alpha * a + beta * b, rtol=1e-15, atol=1e-15)
+    # ComputeResidual: max |a - b| (a maximum does not depend on the order)
+    assert G.residual(da, db) == np.max(np.abs(a - b))
+    assert G.residual(da, db, n=1000) == np.max(np.abs(a[:1000] - b[:1000]))
     res = G.fused_waxpby_dot(-0.5, da, db)
     yb = b - 0.5 * a
     assert np.allclose(db.host(), yb, rtol=1e-15, atol=1e-15)
