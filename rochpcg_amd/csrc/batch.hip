@@ -21,20 +21,6 @@
 namespace hpcgx
 {
 
-__device__ __forceinline__ int find_seg(const BSeg* __restrict__ segs, int nseg, int64_t b)
-{
-    int lo = 0, hi = nseg - 1;
-    while(lo < hi)
-    {
-        const int mid = (lo + hi + 1) >> 1;
-        if(segs[mid].blk0 <= b)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
 // the fold of a reducing launch: block k folds segment k's partials (one per
 // block of the segment) for that segment's problem
 template <int BS>
@@ -417,32 +403,14 @@ struct Planner
             segs.push_back(sg);
             prob.push_back(k);
         }
-        // two rows per thread load R consecutive rows of a slot as one vector,
-        // which needs even offsets, sizes and ELL strides (as fit() in ops.hip)
-        int rows = next_rows;
-        for(const BSeg& sg : segs)
-            if((sg.off % 2) != 0 || (sg.size % 2) != 0 || (sg.m % 2) != 0) rows = 1;
-        int64_t nb = 0;
+        const int rows = launch_rows(segs.data(), segs.size(), next_rows);
+        const int64_t nb = assign_blocks(segs.data(), segs.size(), block, vector_kind, rows);
         for(size_t j = 0; j < segs.size(); ++j)
-        {
-            BSeg& sg = segs[j];
-            if(vector_kind)
-            {
-                int64_t g = ceil_div(sg.size, (int64_t)block * 4);
-                sg.nblk = std::max<int64_t>(1, std::min<int64_t>(g, RED_BLOCKS));
-            }
-            else
-            {
-                sg.nblk = ceil_div(ceil_div(sg.size, (int64_t)rows), (int64_t)block);
-            }
-            if(sg.ra.partial && sg.nblk > B.probs[prob[j]]->partial_cap)
+            if(segs[j].ra.partial && segs[j].nblk > B.probs[prob[j]]->partial_cap)
             {
                 set_error("batched reduction exceeds the partial capacity");
                 return -2;
             }
-            sg.blk0 = nb;
-            nb += sg.nblk;
-        }
         BLaunch L;
         std::memset(&L, 0, sizeof(L));
         L.kind = kind;

@@ -36,6 +36,58 @@ struct BSeg
     RedArgs ra;        // the problem's reduction scratch and scalars
 };
 
+// The segment of launch-relative (logical) block b: the last segment whose
+// first block is <= b.  Segments are numbered in block order (blk0 strictly
+// increasing, every nblk >= 1), so this is the segment holding b.
+__host__ __device__ inline int find_seg(const BSeg* __restrict__ segs, int nseg, int64_t b)
+{
+    int lo = 0, hi = nseg - 1;
+    while(lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if(segs[mid].blk0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Blocks of the segments of one launch, in order: a vector launch gives each
+// segment ceil(size / (4 block)) blocks, at most RED_BLOCKS and at least one;
+// a row launch ceil(ceil(size / rows) / block).  Sets nblk and blk0 and
+// returns the launch's total block count.
+inline int64_t assign_blocks(BSeg* segs, size_t nseg, int block, bool vector_kind, int rows)
+{
+    int64_t nb = 0;
+    for(size_t j = 0; j < nseg; ++j)
+    {
+        BSeg& sg = segs[j];
+        if(vector_kind)
+        {
+            const int64_t g = ceil_div(sg.size, (int64_t)block * 4);
+            sg.nblk = g < 1 ? 1 : (g > RED_BLOCKS ? RED_BLOCKS : g);
+        }
+        else
+        {
+            sg.nblk = ceil_div(ceil_div(sg.size, (int64_t)rows), (int64_t)block);
+        }
+        sg.blk0 = nb;
+        nb += sg.nblk;
+    }
+    return nb;
+}
+
+// Rows per thread a sweep launch can use: 2 loads R consecutive rows of a
+// slot as one vector, which needs even offsets, sizes and ELL strides in
+// every segment; otherwise 1.
+inline int launch_rows(const BSeg* segs, size_t nseg, int rows)
+{
+    for(size_t j = 0; j < nseg; ++j)
+        if((segs[j].off % 2) != 0 || (segs[j].size % 2) != 0 || (segs[j].m % 2) != 0) return 1;
+    return rows;
+}
+
 enum BKind : int
 {
     BK_SPMV = 0,      // y = A x (r -> x), optional p'Ap

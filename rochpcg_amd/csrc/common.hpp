@@ -185,7 +185,7 @@ __device__ __forceinline__ __half ntload(const __half* p)
 // i % 8, so logical block L = base(i % 8) + i / 8 gives each XCD one
 // contiguous slice of the row space (its L2 then serves the x-gathers of
 // neighbouring rows).
-__device__ __forceinline__ int64_t xcd_block(int64_t i, int64_t nblocks)
+__host__ __device__ __forceinline__ int64_t xcd_block(int64_t i, int64_t nblocks)
 {
     const int64_t q = nblocks / NUM_XCD;
     const int64_t rr = nblocks % NUM_XCD;

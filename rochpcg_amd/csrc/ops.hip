@@ -788,6 +788,8 @@ static unsigned red_grid(int64_t n)
     return (unsigned)g;
 }
 
+int64_t reduction_blocks(int64_t n) { return red_grid(n); }
+
 int launch_dot(int64_t n, const void* x, bool x_f64, const void* y, bool y_f64, const RedArgs& ra, hipStream_t s,
                const int* gate)
 {

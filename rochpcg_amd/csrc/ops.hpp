@@ -58,6 +58,9 @@ int launch_restrict(const Level& F, Prec prec, const void* r, bool r_f64, const 
 int launch_prolong(const Level& F, Prec prec, const void* xc, void* x, hipStream_t s, const int* gate);
 int launch_spmv_injected(const Level& F, const double* x, double* y, hipStream_t s);
 int launch_restrict_diff(const Level& F, const double* rf, const double* Axf, double* rc, hipStream_t s);
+// blocks (= partial slots) of a streaming vector reduction over n elements:
+// at most RED_BLOCKS whatever n
+int64_t reduction_blocks(int64_t n);
 int launch_dot(int64_t n, const void* x, bool x_f64, const void* y, bool y_f64, const RedArgs& ra, hipStream_t s,
                const int* gate);
 // *out = max_i |x_i - y_i| (two launches; partial holds >= min(blocks, cap) slots)
