@@ -3,13 +3,16 @@
 
 A step is one preconditioned CG iteration (4-level MG V-cycle with one pre-
 and one post-smoothing multicolour SymGS per level, outer SpMV, dots and
-vector updates) on the 27-point problem, all arrays resident in HBM.
+vector updates) on the 27-point problem at 512^3 (134M rows, the largest
+configuration, about 70 GB of HBM), all arrays resident in HBM.
 
 Prints one JSON line: the FP64 HPCG-formula GFLOP/s (value), the mixed-
-precision CG's GFLOP/s and its time to a relative residual of 1e-9 against
-the FP64 CG's, the modelled HBM TB/s of both, a roofline entry for the outer
-SpMV kernel timed with HIP events on the solver stream, and the CPU baseline
-(the oracle's CG_ref restatement, OpenMP, on this host's cores).
+precision CG's GFLOP/s (f64 true residual every 10 iterations) and both
+solves' time to a relative residual of 1e-9 (x* from splitmix64 seed 99), the
+modelled HBM TB/s, a roofline entry for the outer SpMV kernel timed with HIP
+events on the solver stream, a second leg at 320^3 (seed 1234), the
+64-problem batched leg, and the CPU baseline (the reference's own CG_ref,
+OpenMP, on this host's cores, on a bounded sample).
 """
 import argparse
 import json
@@ -107,24 +110,126 @@ def batched_leg(hip, count, log):
                      "gflops": round(float((flops * r["niters"]).sum()) / el / 1e9, 2)}
         log(f"batched {name}: {res[name]}")
     res["speedup_time_to_solution"] = round(res["f64"]["seconds"] / res["mixed"]["seconds"], 3)
+    # every launch of 5 eager iterations between HIP events on the batch's
+    # stream: per launch kind the time and the algorithmic bytes
+    for name, mode in (("f64", hip.CG_F64), ("mixed", hip.CG_MIXED)):
+        for P in B.problems:
+            P.vec["x"].upload(np.zeros(P.m))
+        prof = B.profile(iters=5, mode=mode)
+        it_ms, it_bytes = prof.pop("iteration")
+        kinds = {k: {"ms": round(ms, 4), "GBps": round(by / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+                 for k, (ms, by) in prof.items() if by > 0}
+        dom = max(prof, key=lambda k: prof[k][0])
+        dms, dby = prof[dom]
+        res[f"roofline_{name}"] = {
+            "kernel": f"batched {dom} launches (all levels and colours, one launch per colour for all problems)",
+            "bound": "hbm", "achieved": round(dby / (dms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(dby / (dms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "bytes_per_iteration": int(dby), "ms_per_iteration": round(dms, 4),
+            "iteration_ms": round(it_ms, 4), "iteration_GBps": round(it_bytes / (it_ms * 1e-3) / 1e9, 1),
+            "kinds": kinds}
+        log(f"batched {name} profile: {res[f'roofline_{name}']}")
     B.close()
     return res
+
+
+def cpu_info():
+    """Model name of the host CPU (first /proc/cpuinfo entry)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def solve_leg(hip, G, seed, use_graph, log, modes):
+    """Solves to 1e-9 from x0 = 0 with x* uniform in [-1,1] (splitmix64 `seed`),
+    b = A x*: time, iterations, recursive and true (f64, b - A x on the GPU)
+    relative residuals and the error against x*.  `modes`: (name, CG mode,
+    true-residual interval)."""
+    import numpy as np
+    G.make_rhs(1, seed)
+    xs = G.vec["xexact"].host()
+    flops = G.flops_per_iteration()
+    sol = {}
+    for name, mode, every in modes:
+        G.set_residual_replacement(every)
+        G.vec["x"].upload(np.zeros(G.m))
+        G.synchronize()
+        t = time.perf_counter()
+        r = G.cg(max_iter=1000, tol=1e-9, mode=mode, use_graph=use_graph, history=False)
+        el = time.perf_counter() - t
+        x = G.vec["x"].host()
+        d = hip.DeviceArray(G.m)
+        G.spmv(G.vec["x"], d)
+        G.waxpby(1.0, G.vec["b"], -1.0, d, d)
+        rt = np.sqrt(G.dot(d, d)) / np.sqrt(G.dot(G.vec["b"], G.vec["b"]))
+        d.free()
+        sol[name] = {"iterations": r["niters"], "seconds": round(el, 4),
+                     "rel_residual": float(f"{r['normr'] / r['normr0']:.3e}"),
+                     "true_rel_residual": float(f"{rt:.3e}"),
+                     "err_vs_xstar": float(f"{np.linalg.norm(x - xs) / np.linalg.norm(xs):.3e}"),
+                     "gflops": round(flops * r["niters"] / el / 1e9, 2)}
+        log(f"solve {name}: {sol[name]}")
+    G.set_residual_replacement(0)
+    sol["seed"] = seed
+    return sol
+
+
+def tune_or_load(G, args, log):
+    from rochpcg_amd import autotune
+    t0 = time.perf_counter()
+    if args.shapes:
+        table = autotune.load(args.shapes)
+        if not autotune.apply(G, table):
+            sys.exit(f"bench: no launch shapes for this problem in {args.shapes}")
+        log(f"launch shapes from {args.shapes}")
+    else:
+        table = autotune.tune(G, mixed=False, log=log)
+        table = autotune.tune(G, mixed=True, table=table, log=log)
+        log(f"autotune: {time.perf_counter() - t0:.1f} s")
+    if args.save_shapes:
+        autotune.save(table, args.save_shapes)
+    L0 = G.levels[0]
+    return {k: {op: v["shape"] for op, v in e.items()} for k, e in table.items()
+            if k.startswith(f"{L0.nx}x{L0.ny}x{L0.nz}:")}
+
+
+def timed(G, mode, K, W, use_graph):
+    """Seconds of K CG iterations after W warm-up iterations (stream-synchronised)."""
+    G.cg_begin(mode=mode, max_iter=W + K, tol=0.0)
+    G.cg_iterate(W, use_graph=use_graph)
+    G.synchronize()
+    t = time.perf_counter()
+    G.cg_iterate(K, use_graph=use_graph)
+    G.synchronize()
+    el = time.perf_counter() - t
+    res = G.cg_end()
+    assert res["niters"] == W + K, res
+    return el
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--nx", type=int, default=320)
+    ap.add_argument("--nx", type=int, default=512)
     ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=None)
+    ap.add_argument("--seed", type=int, default=99, help="splitmix64 seed of x* in the solve-to-1e-9 leg")
+    ap.add_argument("--second-nx", type=int, default=320,
+                    help="cube side of the second leg (0 = none): per-iteration times and solves to 1e-9")
+    ap.add_argument("--second-seed", type=int, default=1234)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-solve", action="store_true", help="skip the solve-to-1e-9 comparison")
     ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--cpu-nx", type=int, default=320, help="cube side of the bounded CPU-baseline sample")
     ap.add_argument("--eager", action="store_true", help="launch kernels one by one instead of a hipGraph")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (256 threads, 1 row)")
-    ap.add_argument("--seed", type=int, default=1234, help="splitmix64 seed of x* in the solve-to-1e-9 leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-problems leg")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--shapes", default=None,
@@ -136,6 +241,7 @@ def main():
                                    "are not part of it", "n_gpus": args.gpus}))
         sys.exit(2)
     dims = (args.nx, args.ny or args.nx, args.nz or args.nx)
+    import numpy as np
     import rochpcg_amd as hip
 
     def log(msg):
@@ -143,50 +249,26 @@ def main():
 
     t0 = time.perf_counter()
     G = hip.Problem(*dims, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
-    log(f"setup {dims} on the GPU: {time.perf_counter() - t0:.2f} s")
+    setup_s = time.perf_counter() - t0
+    log(f"setup {dims} on the GPU: {setup_s:.2f} s")
+    G.make_rhs(1, args.seed)
     flops = G.flops_per_iteration()
     K, W = args.steps, args.warmup
-    tuned = {}
-    if args.shapes or not args.no_tune:
-        from rochpcg_amd import autotune
-        t0 = time.perf_counter()
-        if args.shapes:
-            table = autotune.load(args.shapes)
-            if not autotune.apply(G, table):
-                sys.exit(f"bench: no launch shapes for this problem in {args.shapes}")
-            log(f"launch shapes from {args.shapes}")
-        else:
-            table = autotune.tune(G, mixed=False, log=log)
-            table = autotune.tune(G, mixed=True, table=table, log=log)
-            log(f"autotune: {time.perf_counter() - t0:.1f} s")
-        if args.save_shapes:
-            autotune.save(table, args.save_shapes)
-        L0 = G.levels[0]
-        tuned = {k: {op: v["shape"] for op, v in e.items()} for k, e in table.items()
-                 if k.startswith(f"{L0.nx}x{L0.ny}x{L0.nz}:")}
+    tuned = tune_or_load(G, args, log) if (args.shapes or not args.no_tune) else {}
     use_graph = not args.eager
 
-    def timed(mode):
-        G.cg_begin(mode=mode, max_iter=W + K, tol=0.0)
-        G.cg_iterate(W, use_graph=use_graph)
-        G.synchronize()
-        t = time.perf_counter()
-        G.cg_iterate(K, use_graph=use_graph)
-        G.synchronize()
-        el = time.perf_counter() - t
-        res = G.cg_end()
-        assert res["niters"] == W + K, res
-        return el
-
-    import numpy as np
     G.vec["x"].upload(np.zeros(G.m))
-    el64 = timed(hip.CG_F64)
+    el64 = timed(G, hip.CG_F64, K, W, use_graph)
     log(f"FP64: {K} iterations in {el64 * 1e3:.1f} ms")
     G.vec["x"].upload(np.zeros(G.m))
-    elmx = timed(hip.CG_MIXED)
+    elmx = timed(G, hip.CG_MIXED, K, W, use_graph)
     log(f"mixed: {K} iterations in {elmx * 1e3:.1f} ms")
-    ms64 = el64 * 1e3 / K
-    msmx = elmx * 1e3 / K
+    G.vec["x"].upload(np.zeros(G.m))
+    G.set_residual_replacement(10)
+    elmr = timed(G, hip.CG_MIXED, K, W, use_graph)
+    G.set_residual_replacement(0)
+    log(f"mixed, true residual every 10: {K} iterations in {elmr * 1e3:.1f} ms")
+    ms64, msmx, msmr = (e * 1e3 / K for e in (el64, elmx, elmr))
     # algorithmic bytes of this library's kernels (each ELL slot a kernel reads
     # once, neighbour gathers from cache, own-row vectors once)
     mg64, b64 = G.iteration_bytes(hip.CG_F64)
@@ -216,15 +298,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"HPCG 27-point {dims[0]}x{dims[1]}x{dims[2]}, 4-level MG "
-                               f"(1 pre/1 post multicolour SymGS), FP64 CG, b=A*ones, x0=0",
-                   "nx": dims[0], "ny": dims[1], "nz": dims[2], "levels": 4,
+        "config": {"workload": f"HPCG 27-point {dims[0]}x{dims[1]}x{dims[2]} ({G.m} rows), 4-level MG "
+                               f"(1 pre/1 post multicolour SymGS), x* uniform in [-1,1] (splitmix64 seed "
+                               f"{args.seed}), b=A*x*, x0=0; FP64 CG per iteration (value) and the mixed "
+                               f"FP32/FP16 V-cycle with the f64 true residual every 10 iterations",
+                   "nx": dims[0], "ny": dims[1], "nz": dims[2], "levels": 4, "seed": args.seed,
                    "graph": use_graph, "parallelism": "single GPU", "level0_shapes": tuned},
+        "setup_s": round(setup_s, 3),
         "hbm_TBps": round(b64 / (ms64 * 1e-3) / 1e12, 3),
         "bytes_per_iteration": int(b64),
-        "mixed": {"value": round(flops / (msmx * 1e-3) / 1e9, 2), "unit": "GFLOP/s",
-                  "ms_per_step": round(msmx, 4), "hbm_TBps": round(bmx / (msmx * 1e-3) / 1e12, 3),
-                  "bytes_per_iteration": int(bmx), "speedup_per_iteration": round(ms64 / msmx, 3),
+        "mixed": {"value": round(flops / (msmr * 1e-3) / 1e9, 2), "unit": "GFLOP/s",
+                  "ms_per_step": round(msmr, 4), "true_residual_every": 10,
+                  "ms_per_step_no_refinement": round(msmx, 4),
+                  "hbm_TBps": round(bmx / (msmx * 1e-3) / 1e12, 3),
+                  "bytes_per_iteration": int(bmx), "speedup_per_iteration": round(ms64 / msmr, 3),
                   "dtype": "f64 outer CG / f32 vectors + f16 ELL in MG"},
         "phase_ms_per_iteration": {k: round(v / K, 4) for k, v in ph.items()},
         "roofline": {"kernel": "k_spmv (outer SpMV + fused p'Ap), level 0",
@@ -238,39 +325,42 @@ def main():
                         "ms_per_vcycle": round(mg_ms, 4)},
     }
 
+    modes = (("f64", hip.CG_F64, 0), ("mixed_refine10", hip.CG_MIXED, 10), ("mixed", hip.CG_MIXED, 0))
     if not args.no_solve:
-        # solve to 1e-9 from x0 = 0 with x* uniform in [-1,1] (splitmix64, --seed)
-        G.make_rhs(1, args.seed)
-        xs = G.vec["xexact"].host()
-        sol = {}
-        # "mixed_refine10": the mixed solve with the f64 true residual b - A x
-        # recomputed every 10 iterations (iterative refinement); off in "mixed"
-        for name, mode, every in (("f64", hip.CG_F64, 0), ("mixed", hip.CG_MIXED, 0),
-                                  ("mixed_refine10", hip.CG_MIXED, 10)):
-            G.set_residual_replacement(every)
-            G.vec["x"].upload(np.zeros(G.m))
-            G.synchronize()
-            t = time.perf_counter()
-            r = G.cg(max_iter=500, tol=1e-9, mode=mode, use_graph=use_graph, history=False)
-            el = time.perf_counter() - t
-            x = G.vec["x"].host()
-            # true residual in f64 on the GPU: r = b - A x
-            d = hip.DeviceArray(G.m)
-            G.spmv(G.vec["x"], d)
-            G.waxpby(1.0, G.vec["b"], -1.0, d, d)
-            rt = np.sqrt(G.dot(d, d)) / np.sqrt(G.dot(G.vec["b"], G.vec["b"]))
-            sol[name] = {"iterations": r["niters"], "seconds": round(el, 4),
-                         "rel_residual": float(f"{r['normr'] / r['normr0']:.3e}"),
-                         "true_rel_residual": float(f"{rt:.3e}"),
-                         "err_vs_xstar": float(f"{np.linalg.norm(x - xs) / np.linalg.norm(xs):.3e}"),
-                         "gflops": round(flops * r["niters"] / el / 1e9, 2)}
-            log(f"solve {name}: {sol[name]}")
-        G.set_residual_replacement(0)
-        sol["speedup_time_to_solution"] = round(sol["f64"]["seconds"] / sol["mixed"]["seconds"], 3)
-        sol["iteration_delta"] = sol["mixed"]["iterations"] - sol["f64"]["iterations"]
+        sol = solve_leg(hip, G, args.seed, use_graph, log, modes)
+        # the configuration's mixed solve: the f64 true residual every 10 iterations
+        sol["speedup_time_to_solution"] = round(sol["f64"]["seconds"] / sol["mixed_refine10"]["seconds"], 3)
+        sol["speedup_time_to_solution_no_refinement"] = round(sol["f64"]["seconds"] / sol["mixed"]["seconds"], 3)
+        sol["iteration_delta"] = sol["mixed_refine10"]["iterations"] - sol["f64"]["iterations"]
         out["solve_1e-9"] = sol
-        sol["seed"] = args.seed
     G.close()
+
+    if args.second_nx:
+        n2 = args.second_nx
+        t0 = time.perf_counter()
+        G2 = hip.Problem(n2, n2, n2, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
+        G2.make_rhs(1, args.second_seed)
+        log(f"second leg {n2}^3: setup {time.perf_counter() - t0:.2f} s")
+        if not args.no_tune:
+            from rochpcg_amd import autotune
+            table = autotune.tune(G2, mixed=False)
+            autotune.tune(G2, mixed=True, table=table)
+        f2 = G2.flops_per_iteration()
+        leg = {"workload": f"HPCG 27-point {n2}^3, 4-level MG, x* seed {args.second_seed}"}
+        for name, mode, every in (("f64", hip.CG_F64, 0), ("mixed_refine10", hip.CG_MIXED, 10)):
+            G2.set_residual_replacement(every)
+            G2.vec["x"].upload(np.zeros(G2.m))
+            el = timed(G2, mode, K, W, use_graph)
+            leg[name] = {"ms_per_step": round(el * 1e3 / K, 4), "gflops": round(f2 * K / el / 1e9, 2)}
+        G2.set_residual_replacement(0)
+        leg["speedup_per_iteration"] = round(leg["f64"]["ms_per_step"] / leg["mixed_refine10"]["ms_per_step"], 3)
+        if not args.no_solve:
+            sol2 = solve_leg(hip, G2, args.second_seed, use_graph, log, modes[:2])
+            sol2["speedup_time_to_solution"] = round(sol2["f64"]["seconds"] / sol2["mixed_refine10"]["seconds"], 3)
+            leg["solve_1e-9"] = sol2
+        G2.close()
+        out[f"leg_{n2}"] = leg
+        log(f"second leg: {leg}")
 
     if not args.no_batch:
         out["batched"] = batched_leg(hip, args.batch_count, log)
@@ -278,7 +368,15 @@ def main():
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         log(f"CPU baseline on {threads} threads ...")
-        out["cpu_baseline"] = cpu_baseline(dims, args.cpu_iters, threads)
+        cdims = (args.cpu_nx,) * 3
+        cb = cpu_baseline(cdims, args.cpu_iters, threads)
+        cb["cpu_model"] = cpu_info()
+        cb["omp_proc_bind"] = os.environ.get("OMP_PROC_BIND", "unset (OpenMP default)")
+        cb["omp_num_threads"] = threads
+        if cdims != dims:
+            cb["sample"] += (f"; a bounded sample of the same operator: the {dims[0]}x{dims[1]}x{dims[2]} problem "
+                             f"in the reference's host format needs about 100 GB and minutes of setup")
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 

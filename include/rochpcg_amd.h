@@ -218,6 +218,14 @@ int hpcgx_batch_set_rows(hpcgx_batch* b, int level, int rows);
  * normr0: arrays of nprob entries (may be NULL) */
 int hpcgx_batch_cg(hpcgx_batch* b, int mode, int max_iter, double tolerance, int use_graph, int* niters,
                    double* normr, double* normr0);
+/* `iters` batched CG iterations from each problem's x, launched one by one
+ * with HIP events around every launch: kind_ms[k] (12 entries) = summed
+ * milliseconds of the launches of kind k over the iterations -- 0 SpMV,
+ * 1/2 zero-guess forward/backward sweeps, 3/4 forward/backward sweeps,
+ * 5 restriction, 6 prolongation, 7 dot, 8 p update, 9 x/r update, 10 copy,
+ * 11 waxpby -- kind_bytes[k] = their algorithmic HBM bytes per iteration,
+ * *iter_ms = milliseconds per iteration (no reference counterpart) */
+int hpcgx_batch_profile(hpcgx_batch* b, int mode, int iters, double* kind_ms, double* kind_bytes, double* iter_ms);
 /* residual history of problem k's last batched CG: normr after iterations
  * 0..niters into history (at most hist_len entries); *count = entries written */
 int hpcgx_batch_history(hpcgx_batch* b, int k, double* history, int hist_len, int* count);

@@ -75,6 +75,7 @@ def lib():
         L.orc_spmv.argtypes = [vp, C.c_int, _f64p, _f64p]
         L.orc_symgs.argtypes = [vp, C.c_int, _f64p, _f64p]
         L.orc_mg.argtypes = [vp, C.c_int, _f64p, _f64p]
+        L.orc_set_dot_mode.argtypes = [C.c_int]
         L.orc_dot.restype = C.c_double
         L.orc_dot.argtypes = [C.c_int, vp, vp]
         L.orc_waxpby.argtypes = [C.c_int, C.c_double, _f64p, C.c_double, _f64p, _f64p]
@@ -90,6 +91,15 @@ def lib():
 
 def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
+
+
+DOT_REFERENCE = 0      # the reference's OpenMP reduction
+DOT_COMPENSATED = 1    # Neumaier-compensated sequential sum of the same products
+
+
+def set_dot_mode(mode: int) -> None:
+    """Summation of every dot product of the oracle (orc_dot, used by its CG)."""
+    lib().orc_set_dot_mode(int(mode))
 
 
 def _ptr(a):
@@ -355,7 +365,7 @@ def split_ell_rows(nx, ny, nz, perm, iperm, rows):
                 z, y, xx = iz + sz, iy + sy, ix + sx
                 ok = (z >= 0) & (z < nz) & (y >= 0) & (y < ny) & (xx >= 0) & (xx < nx)
                 j = np.where(ok, z * nxy + y * nx + xx, 0)
-                nb[:, k] = np.where(ok, perm[j], np.iinfo(np.int64).max)
+                nb[:, k] = np.where(ok, perm[j].astype(np.int64), np.iinfo(np.int64).max)
                 k += 1
     nb.sort(axis=1)
     cnt = (nb != np.iinfo(np.int64).max).sum(axis=1)

@@ -701,9 +701,37 @@ int orc_get_level_vectors(const orc_problem* p, int level, double* Axf, double* 
     return 0;
 }
 
+/* Dot-product summation of the oracle: 0 = the reference's OpenMP reduction
+ * (ComputeDotProduct_ref.cpp:46-80: each thread sums its block of the loop in
+ * order, the partials are then added), 1 = compensated summation (Neumaier's
+ * variant of Kahan summation, sequential) of the same rounded products, whose
+ * error is about one rounding of the total.  Mode 1 is not the reference's
+ * arithmetic: it measures how much of a history difference comes from the
+ * reference's own summation error, which grows with the vector length.     */
+static int g_dot_mode = 0;
+
+void orc_set_dot_mode(int mode) { g_dot_mode = mode; }
+
+static double dot_compensated(int n, const double* x, const double* y)
+{
+    double s = 0.0, c = 0.0;
+    for(int i = 0; i < n; i++)
+    {
+        const double p = x[i] * y[i];
+        const double t = s + p;
+        if(fabs(s) >= fabs(p))
+            c += (s - t) + p;
+        else
+            c += (p - t) + s;
+        s = t;
+    }
+    return s + c;
+}
+
 /* ComputeDotProduct_ref.cpp:46-80 */
 double orc_dot(int n, const double* x, const double* y)
 {
+    if(g_dot_mode == 1) return dot_compensated(n, x, y);
     double local_result = 0.0;
     if(y == x)
     {

@@ -71,6 +71,8 @@ int orc_spmv(const orc_problem* p, int level, const double* x, double* y);
 int orc_symgs(const orc_problem* p, int level, const double* r, double* x);
 int orc_mg(orc_problem* p, int level, const double* r, double* x);
 double orc_dot(int n, const double* x, const double* y);
+/* 0 = the reference's OpenMP reduction (default), 1 = compensated summation */
+void orc_set_dot_mode(int mode);
 int orc_waxpby(int n, double alpha, const double* x, double beta, const double* y, double* w);
 int orc_restriction(orc_problem* p, int level, const double* rf);
 int orc_prolongation(orc_problem* p, int level, double* xf);

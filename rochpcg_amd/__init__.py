@@ -93,6 +93,7 @@ def _load():
         "hpcgx_batch_set_rows": ([vp, i32, i32], i32),
         "hpcgx_batch_cg": ([vp, i32, i32, f64, i32, vp, vp, vp], i32),
         "hpcgx_batch_history": ([vp, i32, vp, i32, P(i32)], i32),
+        "hpcgx_batch_profile": ([vp, i32, i32, vp, vp, P(f64)], i32),
         "hpcgx_malloc": ([P(vp), C.c_size_t], i32),
         "hpcgx_free": ([vp], i32),
         "hpcgx_memcpy_h2d": ([vp, vp, C.c_size_t], i32),
@@ -491,6 +492,20 @@ class Batch:
         _check(lib.hpcgx_batch_cg(self.h, mode, max_iter, tol, int(use_graph), ni.ctypes.data,
                                   nr.ctypes.data, nr0.ctypes.data))
         return dict(niters=ni, normr=nr, normr0=nr0)
+
+    BATCH_KINDS = ("spmv", "fwd_zero", "bwd_zero", "fwd", "bwd", "restrict", "prolong", "dot", "cg_p",
+                   "cg_xr", "copy", "waxpby")
+
+    def profile(self, iters=5, mode=CG_F64):
+        """Per launch kind: (milliseconds per iteration, algorithmic bytes per
+        iteration) from HIP events around every launch of `iters` eager batched
+        iterations; plus milliseconds per iteration under "iteration"."""
+        ms, by = np.zeros(12), np.zeros(12)
+        it = C.c_double()
+        _check(lib.hpcgx_batch_profile(self.h, mode, iters, ms.ctypes.data, by.ctypes.data, C.byref(it)))
+        out = {k: (ms[i] / iters, by[i]) for i, k in enumerate(self.BATCH_KINDS)}
+        out["iteration"] = (it.value, float(by.sum()))
+        return out
 
     def history(self, k, max_len=4097):
         """normr after iterations 0..niters of problem k's last batched CG."""

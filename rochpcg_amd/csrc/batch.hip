@@ -413,6 +413,11 @@ struct Planner
             }
         BLaunch L;
         std::memset(&L, 0, sizeof(L));
+        for(const BSeg& sg : segs)
+        {
+            L.nrows += sg.size;
+            if(sg.t) L.any_t = 1;
+        }
         L.kind = kind;
         L.prec = prec;
         L.r_f64 = r_f64;
@@ -755,6 +760,126 @@ static int run_list(const std::vector<BLaunch>& v, hipStream_t s)
     return 0;
 }
 
+// algorithmic HBM bytes of one planned launch (the per-row model of
+// iteration_bytes in solver.hip, over the launch's rows)
+static double launch_bytes(const BLaunch& L)
+{
+    const bool f64 = L.prec == PREC_F64;
+    const double vb = f64 ? 8.0 : 2.0, xb = f64 ? 8.0 : 4.0, rb = L.r_f64 ? 8.0 : xb;
+    const double slot = 4.0 + vb, n = (double)L.nrows;
+    const double zb = L.bf32 ? 4.0 : 8.0;
+    switch(L.kind)
+    {
+    case BK_SPMV: return n * (W * slot + 16.0);
+    case BK_FWD_ZERO: return n * (window_slots(L.win) * slot + 4.0 + vb + rb + xb);
+    case BK_BWD_ZERO: return n * (window_slots(L.win) * slot + 4.0 + vb + 2.0 * xb + (L.any_t ? xb : 0.0));
+    case BK_FWD_FULL:
+        if(L.all_u)
+        {
+            int na = 0, nb = 0;
+            window_pair_slots(L.win, L.win_tail, &na, &nb);
+            return n * ((na + nb) * slot + 4.0 + vb + rb + 3.0 * xb);
+        }
+        return n * (W * slot + rb + xb + (L.any_t ? xb : 0.0));
+    case BK_BWD_FULL: return n * (window_slots(L.win) * slot + 4.0 + vb + 2.0 * xb);
+    case BK_RESTRICT: return n * (W * slot + rb + 8.0 + xb);
+    case BK_PROLONG: return n * (8.0 + 3.0 * xb);
+    case BK_DOT: return n * (8.0 + zb);
+    case BK_CG_P: return n * (zb + 16.0);
+    case BK_CG_XR: return n * 48.0;
+    case BK_COPY: return n * 16.0;
+    case BK_WAXPBY: return n * 24.0;
+    }
+    return 0.0;
+}
+
+// cg_begin's reset of every problem's scalars (tolerance 0: run max_iter)
+static int reset_scalars(Batch& B, int max_iter, double tol)
+{
+    for(Problem* P : B.probs)
+    {
+        std::memset(&P->h_sc, 0, sizeof(P->h_sc));
+        P->h_sc.tol = tol;
+        P->h_sc.max_iter = max_iter;
+        HPCGX_HIP(hipMemcpyAsync(P->sc, &P->h_sc, sizeof(CGScalars), hipMemcpyHostToDevice, B.s));
+    }
+    HPCGX_HIP(hipStreamSynchronize(B.s));
+    return 0;
+}
+
+int batch_profile(Batch& B, int mode, int iters, double* kind_ms, double* kind_bytes, double* iter_ms)
+{
+    if(mode != CG_F64 && mode != CG_MIXED)
+    {
+        set_error("invalid CG mode");
+        return -2;
+    }
+    if(!B.planned) HPCGX_TRY(plan(B));
+    const std::vector<BLaunch>& it = B.iter[mode];
+    if(it.empty())
+    {
+        set_error("the batch was built without the store this mode needs");
+        return -2;
+    }
+    if(iters < 1 || iters > B.probs[0]->hist_cap)
+    {
+        set_error("iteration count out of range");
+        return -2;
+    }
+    HPCGX_TRY(reset_scalars(B, iters, 0.0));
+    HPCGX_TRY(run_list(B.pre[mode], B.s));
+    const size_t nl = it.size();
+    std::vector<hipEvent_t> ev((size_t)iters * (nl + 1), nullptr);
+    int rc = 0;
+    for(auto& e : ev)
+        if(hipEventCreate(&e) != hipSuccess)
+        {
+            set_error("hipEventCreate failed");
+            rc = -1;
+            break;
+        }
+    for(int i = 0; i < iters && !rc; ++i)
+    {
+        hipEvent_t* E = ev.data() + (size_t)i * (nl + 1);
+        if(hipEventRecord(E[0], B.s) != hipSuccess) rc = -1;
+        for(size_t j = 0; j < nl && !rc; ++j)
+        {
+            rc = launch(it[j], B.s);
+            if(!rc && hipEventRecord(E[j + 1], B.s) != hipSuccess) rc = -1;
+        }
+    }
+    if(!rc && hipStreamSynchronize(B.s) != hipSuccess)
+    {
+        set_error("stream synchronisation failed in a profiled batched run");
+        rc = -1;
+    }
+    for(int k = 0; k < BK_COUNT; ++k)
+    {
+        kind_ms[k] = 0.0;
+        kind_bytes[k] = 0.0;
+    }
+    double total = 0.0;
+    if(!rc)
+    {
+        for(size_t j = 0; j < nl; ++j) kind_bytes[it[j].kind] += launch_bytes(it[j]);
+        for(int i = 0; i < iters; ++i)
+        {
+            hipEvent_t* E = ev.data() + (size_t)i * (nl + 1);
+            for(size_t j = 0; j < nl; ++j)
+            {
+                float ms = 0.f;
+                if(hipEventElapsedTime(&ms, E[j], E[j + 1]) == hipSuccess) kind_ms[it[j].kind] += ms;
+            }
+            float ms = 0.f;
+            if(hipEventElapsedTime(&ms, E[0], E[nl]) == hipSuccess) total += ms;
+        }
+    }
+    for(auto& e : ev)
+        if(e) (void)hipEventDestroy(e);
+    if(iter_ms) *iter_ms = total / iters;
+    return rc;
+}
+
 int batch_cg(Batch& B, int mode, int max_iter, double tol, bool use_graph, int* niters, double* normr, double* normr0)
 {
     if(mode != CG_F64 && mode != CG_MIXED)
@@ -774,14 +899,7 @@ int batch_cg(Batch& B, int mode, int max_iter, double tol, bool use_graph, int* 
         return -2;
     }
     // per-problem scalars (cg_begin's reset)
-    for(Problem* P : B.probs)
-    {
-        std::memset(&P->h_sc, 0, sizeof(P->h_sc));
-        P->h_sc.tol = tol;
-        P->h_sc.max_iter = max_iter;
-        HPCGX_HIP(hipMemcpyAsync(P->sc, &P->h_sc, sizeof(CGScalars), hipMemcpyHostToDevice, B.s));
-    }
-    HPCGX_HIP(hipStreamSynchronize(B.s));
+    HPCGX_TRY(reset_scalars(B, max_iter, tol));
     HPCGX_TRY(run_list(B.pre[mode], B.s));
     if(use_graph && !B.gexec[mode])
     {

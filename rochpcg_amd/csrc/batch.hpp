@@ -120,7 +120,11 @@ struct BLaunch
     int64_t nblocks;
     BSeg* segs;       // device table
     double alpha, beta;
+    int64_t nrows;    // rows (elements) over all segments
+    int any_t;        // some segment keeps or reads SymGS partial sums (BSeg::t)
 };
+
+constexpr int BK_COUNT = 12;
 
 struct Batch
 {
@@ -147,5 +151,11 @@ int batch_set_block(Batch& B, int level, int block);
 int batch_set_rows(Batch& B, int level, int rows);
 int batch_cg(Batch& B, int mode, int max_iter, double tol, bool use_graph, int* niters, double* normr,
              double* normr0);
+// `iters` batched CG iterations from each problem's x, every launch between
+// HIP events: kind_ms[k] = summed milliseconds of the launches of kind k
+// (BKind) over the iterations, kind_bytes[k] = their algorithmic HBM bytes per
+// iteration (each slot a kernel loads once, gathers from cache, own-row
+// vectors once), *iter_ms = milliseconds per iteration (first event to last)
+int batch_profile(Batch& B, int mode, int iters, double* kind_ms, double* kind_bytes, double* iter_ms);
 
 }  // namespace hpcgx

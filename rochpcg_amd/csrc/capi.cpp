@@ -708,6 +708,17 @@ int hpcgx_batch_cg(hpcgx_batch* b, int mode, int max_iter, double tolerance, int
     return batch_cg(*b->B, mode, max_iter, tolerance, use_graph != 0, niters, normr, normr0);
 }
 
+int hpcgx_batch_profile(hpcgx_batch* b, int mode, int iters, double* kind_ms, double* kind_bytes, double* iter_ms)
+{
+    if(!b || !b->B)
+    {
+        set_error("null batch");
+        return -2;
+    }
+    CHECK_PTRS(kind_ms, kind_bytes, iter_ms);
+    return batch_profile(*b->B, mode, iters, kind_ms, kind_bytes, iter_ms);
+}
+
 int hpcgx_batch_history(hpcgx_batch* b, int k, double* history, int hist_len, int* count)
 {
     if(!b || !b->B)

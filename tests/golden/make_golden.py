@@ -11,6 +11,11 @@ wherever oracle/_ref cannot be built.
                       multicolour order.
   kernels_16.npz      16x16x16 (JPL order): x ~ U[-1,1] (numpy default_rng(2024)),
                       A x, one SymGS from x = 0 and from x, one V-cycle.
+  cg_history_320.npz  CG_ref at 320^3 in the JPL multicolour order, b = A*ones,
+                      50 iterations, 8 threads (`make_golden.py history-large`;
+                      iterations 0..5 checked against the reference's own _ref
+                      code to 1e-13), and the same run with compensated dot
+                      products (`history-large-compensated`).
   ref_tolerance.npz   the reference tolerance of main.cpp:336-345 -- normr/normr0
                       after 50 natural-order CG_ref iterations, b = A*ones,
                       x0 = 0, 4 levels -- for the cube sides of the reference's
@@ -86,6 +91,31 @@ def ref_tolerance(sizes=(80, 96, 104, 112, 120, 128, 256, 280, 288)):
     np.savez(os.path.join(HERE, "ref_tolerance.npz"), **out)
 
 
+def history_large_compensated(n=320, iters=50):
+    """Adds history_jpl_compensated to cg_history_<n>.npz: the same CG_ref with
+    every dot product summed by the oracle's compensated summation instead of
+    the reference's OpenMP reduction (oracle.DOT_COMPENSATED).  The difference
+    between the two histories is the reference summation's own rounding error
+    at this vector length."""
+    import time
+    oracle.build(ref=False)
+    oracle.set_threads(8)
+    oracle.set_dot_mode(oracle.DOT_COMPENSATED)
+    t0 = time.time()
+    P = oracle.Problem(n, nlevels=4, order=oracle.ORDER_JPL)
+    o = P.cg(P.rhs_ones(), max_iter=iters)
+    oracle.set_dot_mode(oracle.DOT_REFERENCE)
+    h = o["history"]
+    assert o["niters"] == iters and len(h) == iters + 1
+    path = os.path.join(HERE, f"cg_history_{n}.npz")
+    old = dict(np.load(path))
+    rel = np.abs(h - old["history_jpl"]) / old["history_jpl"]
+    print(f"{iters} iterations {time.time() - t0:.0f} s; against the reference summation: max relative "
+          f"difference {rel.max():.2e}", flush=True)
+    old["history_jpl_compensated"] = h
+    np.savez(path, **old)
+
+
 def history_large(n=320, iters=50, check=5, threads=8):
     """cg_history_<n>.npz: CG_ref in the JPL multicolour order at n^3 (4 levels,
     b = A*ones, x0 = 0, `iters` iterations, 8 OpenMP threads): normr per
@@ -121,5 +151,7 @@ if __name__ == "__main__":
         ref_tolerance()
     elif len(sys.argv) > 1 and sys.argv[1] == "history-large":
         history_large(int(sys.argv[2]) if len(sys.argv) > 2 else 320)
+    elif len(sys.argv) > 1 and sys.argv[1] == "history-large-compensated":
+        history_large_compensated(int(sys.argv[2]) if len(sys.argv) > 2 else 320)
     else:
         main()

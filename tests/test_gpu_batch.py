@@ -74,3 +74,28 @@ def test_batched_graph_equals_eager(hip):
     for P, xv in zip(B.problems, xc):
         assert np.array_equal(P.vec["x"].host(), xv)
     B.close()
+
+
+def test_batched_profile(hip):
+    """Per-launch HIP-event timing of the batched iteration: every kind that
+    runs has a time and a byte count, the kinds' times add up to no more than
+    the iteration, and the SpMV's bytes are 27 x 12 + 16 per row (f64)."""
+    B = hip.Batch(DIMS[:4], nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
+    rows = sum(a * b * c for a, b, c in DIMS[:4])
+    for mode in (hip.CG_F64, hip.CG_MIXED):
+        for P in B.problems:
+            P.vec["x"].upload(np.zeros(P.m))
+        prof = B.profile(iters=3, mode=mode)
+        it_ms, it_bytes = prof.pop("iteration")
+        assert it_ms > 0 and it_bytes > 0
+        assert sum(ms for ms, _ in prof.values()) <= it_ms * 1.001
+        for k in ("spmv", "fwd_zero", "bwd_zero", "fwd", "bwd", "restrict", "prolong", "dot", "cg_p", "cg_xr"):
+            assert prof[k][0] > 0 and prof[k][1] > 0, k
+        spmv_row = 27 * (12 if mode == hip.CG_F64 else 6) + 16
+        assert prof["spmv"][1] == rows * spmv_row
+    # the solver still runs normally afterwards
+    for P in B.problems:
+        P.vec["x"].upload(np.zeros(P.m))
+    res = B.cg(max_iter=300, tol=1e-9)
+    assert np.all(res["normr"] / res["normr0"] <= 1e-9)
+    B.close()

@@ -2,7 +2,8 @@
 
   320^3 FP64       50-iteration residual history against the oracle's CG_ref in
                    the same colour order (tests/golden/cg_history_320.npz),
-                   1e-12 relative on every iteration.
+                   1e-12 relative on every iteration (dot products summed with
+                   compensation in the oracle).
   320^3 mixed      x* uniform in [-1,1] (splitmix64 seed 1234), true residual
                    every 10 iterations, solved to 1e-9 next to the FP64 solve.
   64 uneven        nx, ny, nz each from {16, 24, ..., 96} (splitmix64 seed 7),
@@ -46,22 +47,31 @@ def _solution_checks(oracle_mod, G, perm, dims):
 
 
 def test_config_fp64_320_history(hip):
-    """FP64 CG at 320^3, b = A*ones, x0 = 0, 50 iterations: every iteration's
-    residual norm within 1e-12 relative of the oracle's CG_ref in the same
-    colour order.  The 50 iterations reduce the residual by only ~4e-3, so
-    the relative bar holds without any allowance for rounding noise."""
+    """FP64 CG at 320^3, b = A*ones, x0 = 0, 50 iterations, against the
+    oracle's CG_ref in the same colour order (8 threads).  The 50 iterations
+    reduce the residual by only ~4e-3, so no allowance for rounding noise is
+    made anywhere: every iteration within 1e-12 relative of the oracle run
+    whose dot products are summed with compensation (the GPU's two-level tree
+    sums are accurate to ~1e-15 at this length).  The reference's own OpenMP
+    reduction -- each thread summing 4 M products in order -- is off by up to
+    ~4e-11 relative at this length (the two oracle histories differ by that
+    much); against it the bar is that reference summation error plus 1e-12."""
     g = np.load(os.path.join(GOLDEN, "cg_history_320.npz"))
-    hr = g["history_jpl"]
+    hr, hc = g["history_jpl"], g["history_jpl_compensated"]
+    own = np.abs(hr - hc) / hc          # the reference summation's rounding error
     _say("config fp64 320^3: 50 iterations against the oracle history")
     G = hip.Problem(320, nlevels=4, stores=hip.STORE_F64)
     res = G.cg(max_iter=50, tol=0.0, use_graph=True)
     G.close()
     h = res["history"]
     assert res["niters"] == 50 and len(h) == 51
-    rel = np.abs(h - hr) / hr
-    _say(f"320^3 history: max rel {rel.max():.2e} (iteration {int(rel.argmax())}), "
+    rel_c = np.abs(h - hc) / hc
+    rel_r = np.abs(h - hr) / hr
+    _say(f"320^3 history: max rel {rel_c.max():.2e} against compensated dots (iteration {int(rel_c.argmax())}), "
+         f"{rel_r.max():.2e} against the reference's summation (its own error {own.max():.2e}); "
          f"reduction {h[-1] / h[0]:.4e}")
-    assert np.all(rel <= 1e-12), rel
+    assert np.all(rel_c <= 1e-12), rel_c
+    assert np.all(rel_r <= own + 1e-12), (rel_r, own)
 
 
 def test_config_mixed_320(hip, oracle_mod):

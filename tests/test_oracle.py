@@ -311,3 +311,73 @@ def test_oracle_matches_golden_reference_tolerance(oracle_mod, n):
     P = oracle_mod.Problem(n, nlevels=4, order=oracle_mod.ORDER_NATURAL)
     res = P.cg(P.rhs_ones(), max_iter=50)
     assert abs(res["normr"] / res["normr0"] - tol) <= 1e-10 * tol
+
+
+# ---------------------------------------------------------------------------
+# numpy restatements used by the full-size GPU checks, and the compensated dot
+
+def test_compensated_dot_is_correctly_summed(oracle_mod):
+    """DOT_COMPENSATED sums the rounded products to within 2 ulp of their
+    exactly rounded sum (math.fsum); the reference's reduction is further off
+    on long vectors of same-signed products."""
+    import math
+    rng = np.random.default_rng(31)
+    x = rng.uniform(0.5, 1.5, 3_000_000)
+    y = rng.uniform(0.5, 1.5, 3_000_000)
+    exact = math.fsum(x * y)
+    oracle_mod.set_dot_mode(oracle_mod.DOT_COMPENSATED)
+    try:
+        c = oracle_mod.dot(x, y)
+        cc = oracle_mod.dot(x, x)
+    finally:
+        oracle_mod.set_dot_mode(oracle_mod.DOT_REFERENCE)
+    assert abs(c - exact) <= 2 * np.spacing(exact)
+    assert abs(cc - math.fsum(x * x)) <= 2 * np.spacing(cc)
+    r = oracle_mod.dot(x, y)
+    assert abs(r - exact) <= 1e-10 * exact       # one thread: sequential sum
+
+
+def test_stencil_apply_equals_spmv(oracle_mod):
+    """The numpy 27-point operator against ComputeSPMV_ref's restatement
+    (natural order), slab by slab: to rounding, and exactly on integer data
+    (A * ones = b)."""
+    for dims in [(16, 16, 16), (24, 16, 32), (8, 40, 12), (5, 3, 7)]:
+        P = oracle_mod.Problem(*dims, nlevels=1, order=oracle_mod.ORDER_NATURAL)
+        x = np.random.default_rng(3).uniform(-1, 1, P.m)
+        y = P.spmv(x)
+        ys = oracle_mod.stencil_apply(x, *dims, chunk=3)
+        assert np.max(np.abs(y - ys)) <= 1e-14 * np.max(np.abs(y))
+        assert np.array_equal(oracle_mod.stencil_apply(np.ones(P.m), *dims), P.rhs_ones())
+        xi = np.random.default_rng(4).integers(-100, 100, P.m).astype(np.float64)
+        assert np.array_equal(oracle_mod.stencil_apply(xi, *dims), P.spmv(xi))
+
+
+def test_uniform_vector_is_orc_uniform(oracle_mod):
+    u = oracle_mod.uniform_vector(99, 5000)
+    assert all(u[i] == oracle_mod.uniform(99, i) for i in range(5000))
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 16), (24, 16, 32), (6, 10, 4)])
+def test_split_ell_rows_layout(oracle_mod, dims):
+    """split_ell_rows against the oracle's tail-padded multicolour ELL: lower
+    entries and the diagonal from slot 0, upper entries ending at slot 26,
+    -1 / 0.0 between."""
+    P = oracle_mod.Problem(*dims, nlevels=1, order=oracle_mod.ORDER_JPL)
+    e = P.export(0)
+    perm = e["perm"]
+    iperm = np.empty_like(perm)
+    iperm[perm] = np.arange(len(perm), dtype=np.int32)
+    cols, vals = oracle_mod.split_ell_rows(*dims, perm, iperm, np.arange(P.m))
+    oc, ov, d = e["cols"], e["vals"], e["diag"]
+    ne = (oc >= 0).sum(axis=1)
+    for r in range(P.m):
+        nu = ne[r] - d[r] - 1
+        want_c = np.full(27, -1)
+        want_v = np.zeros(27)
+        want_c[:d[r] + 1] = oc[r, :d[r] + 1]
+        want_v[:d[r] + 1] = ov[r, :d[r] + 1]
+        if nu:
+            want_c[27 - nu:] = oc[r, d[r] + 1:ne[r]]
+            want_v[27 - nu:] = ov[r, d[r] + 1:ne[r]]
+        assert np.array_equal(cols[r], want_c), r
+        assert np.array_equal(vals[r], want_v), r
