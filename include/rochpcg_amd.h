@@ -117,6 +117,16 @@ int hpcgx_restriction_axf(hpcgx_problem* p, int level, const double* rf, const d
 int hpcgx_prolongation(hpcgx_problem* p, int level, double* xf, double** xc_out);
 /* ComputeDotProduct(n, x, y, result, ...)   ComputeDotProduct.cpp:189-238 */
 int hpcgx_dot(hpcgx_problem* p, int64_t n, const double* x, const double* y, double* result);
+/* hpcgx_dot without a problem handle and without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels as hpcgx_dot, with the block partials and the
+ * result in a fixed 33 KB device array of the library (4096 partials; the
+ * reduction never uses more, whatever n); a call that would need more is
+ * refused.  Runs on `stream` (NULL: the null stream) of the current device
+ * and synchronises it; *result is a host double.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU race. */
+int hpcgx_dot_noalloc(int64_t n, const double* x, const double* y, double* result, void* stream);
 /* ComputeResidual(n, v1, v2, residual): max_i |v1_i - v2_i|
  * ComputeResidual.cpp:121-152 (synchronises, like the reference) */
 int hpcgx_residual(hpcgx_problem* p, int64_t n, const double* v1, const double* v2, double* result);

@@ -64,6 +64,7 @@ def _load():
         "hpcgx_restriction_axf": ([vp, i32, vp, vp, P(vp)], i32),
         "hpcgx_residual": ([vp, i64, vp, vp, P(f64)], i32),
         "hpcgx_dot": ([vp, i64, vp, vp, P(f64)], i32),
+        "hpcgx_dot_noalloc": ([i64, vp, vp, P(f64), vp], i32),
         "hpcgx_waxpby": ([vp, i64, f64, vp, f64, vp, vp], i32),
         "hpcgx_fused_waxpby_dot": ([vp, i64, f64, vp, vp, P(f64)], i32),
         "hpcgx_copy_diagonal": ([vp, i32, vp], i32),
@@ -117,6 +118,15 @@ EXPORTED = tuple(n for n in dir(lib) if n.startswith("hpcgx_"))
 def _check(rc):
     if rc != 0:
         raise HPCGError(lib.hpcgx_last_error().decode() or f"status {rc}")
+
+
+def dot_noalloc(x, y, n, stream=None) -> float:
+    """x'y over n doubles without a problem handle or any device allocation
+    (hpcgx_dot_noalloc: the reduction scratch is a fixed device array of the
+    library, shared by every call on the GPU)."""
+    res = C.c_double()
+    _check(lib.hpcgx_dot_noalloc(int(n), _p(x), _p(y), C.byref(res), stream))
+    return res.value
 
 
 def device_count() -> int:

@@ -308,6 +308,13 @@ int hpcgx_dot(hpcgx_problem* p, int64_t n, const double* x, const double* y, dou
     return 0;
 }
 
+int hpcgx_dot_noalloc(int64_t n, const double* x, const double* y, double* result, void* stream)
+{
+    CHECK_PTRS(result);
+    if(n > 0) CHECK_PTRS(x, y);
+    return dot_noalloc(n, x, y, result, (hipStream_t)stream);
+}
+
 int hpcgx_residual(hpcgx_problem* p, int64_t n, const double* v1, const double* v2, double* result)
 {
     CHECK_P(p);

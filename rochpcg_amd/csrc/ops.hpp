@@ -63,6 +63,9 @@ int launch_restrict_diff(const Level& F, const double* rf, const double* Axf, do
 int64_t reduction_blocks(int64_t n);
 int launch_dot(int64_t n, const void* x, bool x_f64, const void* y, bool y_f64, const RedArgs& ra, hipStream_t s,
                const int* gate);
+// the dot product over a file-scope device scratch array (noalloc.hip): no
+// device allocation; *result (host) is ready on return
+int dot_noalloc(int64_t n, const double* x, const double* y, double* result, hipStream_t s);
 // *out = max_i |x_i - y_i| (two launches; partial holds >= min(blocks, cap) slots)
 int launch_absdiff_max(int64_t n, const double* x, const double* y, double* partial, int64_t cap, double* out,
                        hipStream_t s);
