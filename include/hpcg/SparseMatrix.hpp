@@ -59,8 +59,10 @@ struct SparseMatrix_STRUCT
     global_int_t* d_localToGlobalMap;
     local_int_t* d_rowHash;
 
-    // ELL storage (column-major: entry p of row r at p*localNumberOfRows + r,
-    // 64-bit offsets), multicolour row order
+    // ELL storage, multicolour row order, sliced: rows in slices of 128, slice
+    // s holding slot p of its rows at [(s*ell_width + p)*128, +128), so entry p
+    // of row r sits at ((r/128)*ell_width + p)*128 + r%128 (64-bit offsets;
+    // the reference's ELL is column-major, p*localNumberOfRows + r)
     local_int_t ell_width;
     local_int_t* ell_col_ind;
     double* ell_val;

@@ -34,6 +34,9 @@ typedef struct hpcgx_problem hpcgx_problem;
 #define HPCGX_STORE_F64 1   /* f64 ELL values: the reference's arithmetic            */
 #define HPCGX_STORE_MIXED 2 /* f16 ELL values + f32 MG vectors (mixed-precision CG)   */
 
+/* rows per slice of the sliced ELL (hpcgx_level_arrays) */
+#define HPCGX_ELL_SLICE 128
+
 /* CG modes */
 #define HPCGX_CG_F64 0      /* everything in f64                                      */
 #define HPCGX_CG_MIXED 1    /* f64 outer CG, f32/f16 V-cycle, true residual every k   */
@@ -63,8 +66,11 @@ int hpcgx_level_info(const hpcgx_problem* p, int level, int64_t* info);
 int hpcgx_color_table(const hpcgx_problem* p, int level, int* sizes, int* offsets, int* round_sizes);
 /* device pointers of a level's arrays (SparseMatrix::ell_col_ind, ell_val,
  * diag_idx, inv_diag, perm; MGData::d_f2cOperator); any output may be NULL.
- * The ELL is column-major (entry p of row r at p*rows + r) with the entries of
- * each row in the reference's ascending column order; the padding (-1, 0.0)
+ * The ELL is sliced: rows in slices of HPCGX_ELL_SLICE = 128, slice s holding
+ * slot p of its rows at [(s*27 + p)*128, +128), i.e. entry p of row r at
+ * ((r/128)*27 + p)*128 + r%128, over rows rounded up to a whole slice (the
+ * reference's ELL is column-major, p*rows + r; SparseMatrix.cpp:271-407).  The
+ * entries of each row are in the reference's ascending column order; the padding (-1, 0.0)
  * sits between the diagonal and the upper entries, which end at slot 26,
  * instead of at the row's tail (SparseMatrix.cpp:271-407 pads the tail).
  * ptrs must hold 13 entries: ptrs[0..12] = ell_col_ind, ell_val (f64 or

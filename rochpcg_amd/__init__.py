@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(HERE, "librochpcg_amd.so")
 
 STORE_F64 = 1
 STORE_MIXED = 2
+ELL_SLICE = 128   # rows per slice of the sliced ELL (HPCGX_ELL_SLICE)
 CG_F64 = 0
 CG_MIXED = 1
 
@@ -187,6 +188,29 @@ class DeviceArray:
             pass
 
 
+def ell_padded_rows(m):
+    """Rows the sliced ELL arrays of an m-row level hold (m rounded up to a
+    whole slice)."""
+    return -(-int(m) // ELL_SLICE) * ELL_SLICE
+
+
+def ell_unslice(flat, m):
+    """A sliced ELL array (slice s holds slot p of its rows at
+    [(s*27 + p)*ELL_SLICE, +ELL_SLICE)) as an (m, 27) array, row by row."""
+    ns = ell_padded_rows(m) // ELL_SLICE
+    return flat.reshape(ns, 27, ELL_SLICE).transpose(0, 2, 1).reshape(ns * ELL_SLICE, 27)[:m]
+
+
+def ell_rows(view, r0, count):
+    """Rows [r0, r0 + count) of a device ELL array view as a (count, 27)
+    host array (reads only the slices that hold them)."""
+    s0, s1 = r0 // ELL_SLICE, -(-(r0 + count) // ELL_SLICE)
+    blk = view.host(s0 * 27 * ELL_SLICE, (s1 - s0) * 27 * ELL_SLICE)
+    rows = blk.reshape(s1 - s0, 27, ELL_SLICE).transpose(0, 2, 1).reshape(-1, 27)
+    a = r0 - s0 * ELL_SLICE
+    return rows[a:a + count]
+
+
 def _p(a):
     if a is None:
         return None
@@ -258,19 +282,25 @@ class Problem:
         return sizes, offsets, rounds
 
     def level_arrays(self, level=0):
-        """Device arrays of a level copied to host numpy arrays."""
+        """Device arrays of a level copied to host numpy arrays; the ELL
+        arrays as (rows, 27), row by row."""
         L = self.levels[level]
         ptrs = (C.c_void_p * 13)()
         _check(lib.hpcgx_level_arrays(self.h, level, ptrs))
         m, mc = L.m, L.mc
+        me = ell_padded_rows(m)
 
         def get(ptr, n, dt):
             if not ptr or n == 0:
                 return None
             return DeviceArray(n, dt, ptr=ptr, owner=False).host()
+
+        def ell(ptr, dt):
+            a = get(ptr, 27 * me, dt)
+            return None if a is None else ell_unslice(a, m)
         out = dict(
-            ell_col=get(ptrs[0], 27 * m, np.int32), ell_val=get(ptrs[1], 27 * m, np.float64),
-            ell_f16=get(ptrs[2], 27 * m, np.float16), diag_idx=get(ptrs[3], m, np.int32),
+            ell_col=ell(ptrs[0], np.int32), ell_val=ell(ptrs[1], np.float64),
+            ell_f16=ell(ptrs[2], np.float16), diag_idx=get(ptrs[3], m, np.int32),
             inv_diag=get(ptrs[4], m, np.float64), perm=get(ptrs[5], m, np.int32),
             colors=get(ptrs[6], m, np.int32), f2c=get(ptrs[7], mc, np.int32),
             f2c_fine=get(ptrs[8], mc, np.int32), f2c_coarse=get(ptrs[9], mc, np.int32),
@@ -280,13 +310,14 @@ class Problem:
 
     def level_views(self, level=0):
         """Device arrays of a level as non-owning DeviceArray views (no copy);
-        the ELL arrays are column-major [27, rows]."""
+        the ELL arrays are sliced (read rows with ell_rows)."""
         L = self.levels[level]
         ptrs = (C.c_void_p * 13)()
         _check(lib.hpcgx_level_arrays(self.h, level, ptrs))
         m, mc = L.m, L.mc
-        spec = (("ell_col", 27 * m, np.int32), ("ell_val", 27 * m, np.float64),
-                ("ell_f16", 27 * m, np.float16), ("diag_idx", m, np.int32), ("inv_diag", m, np.float64),
+        me = ell_padded_rows(m)
+        spec = (("ell_col", 27 * me, np.int32), ("ell_val", 27 * me, np.float64),
+                ("ell_f16", 27 * me, np.float16), ("diag_idx", m, np.int32), ("inv_diag", m, np.float64),
                 ("perm", m, np.int32), ("colors", m, np.int32), ("f2c", mc, np.int32),
                 ("f2c_fine", mc, np.int32), ("f2c_coarse", mc, np.int32), ("desc", m, np.int32),
                 ("diag", m, np.float64), ("diag_f16", m, np.float16))

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(BS) void kb_spmv(const BSeg* __restrict__ segs, int
     {
         const int64_t row = S.off + i;
         XT s[1];
-        row_spmv<VT, XT, 1>(S.m, row, S.col, (const VT*)S.val, (const XT*)S.r, s);
+        row_spmv<VT, XT, 1>(row, S.col, (const VT*)S.val, (const XT*)S.r, s);
         ((YT*)S.x)[row] = (YT)s[0];
         if(DOT) acc = fmaT((double)((const XT*)S.r)[row], (double)s[0], acc);
     }
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(BS) void kb_fwd_zero(const BSeg* __restrict__ segs,
     BSEG();
     const int64_t i = (lb * BS + threadIdx.x) * R;
     if(i >= S.size) return;
-    row_fwd_zero<NS, VT, XT, RT, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+    row_fwd_zero<NS, VT, XT, RT, R>(S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
                                     (const RT*)S.r, (XT*)S.x);
 }
 
@@ -74,10 +74,10 @@ __global__ __launch_bounds__(BS) void kb_bwd_zero(const BSeg* __restrict__ segs,
     const int64_t i = (lb * BS + threadIdx.x) * R;
     if(i >= S.size) return;
     if(S.t)
-        row_bwd_zero<NS, VT, XT, true, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+        row_bwd_zero<NS, VT, XT, true, R>(S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
                                           (XT*)S.x, (XT*)S.t);
     else
-        row_bwd_zero<NS, VT, XT, false, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+        row_bwd_zero<NS, VT, XT, false, R>(S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
                                            (XT*)S.x, nullptr);
 }
 
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(BS) void kb_fwd_full_u(const BSeg* __restrict__ seg
     BSEG();
     const int64_t i = (lb * BS + threadIdx.x) * R;
     if(i >= S.size) return;
-    row_fwd_full_u<NA, NB, VT, XT, RT, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+    row_fwd_full_u<NA, NB, VT, XT, RT, R>(S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
                                           (const RT*)S.r, (XT*)S.x, (XT*)S.t);
 }
 
@@ -99,13 +99,13 @@ __global__ __launch_bounds__(BS) void kb_fwd_full(const BSeg* __restrict__ segs,
     const int64_t i = (lb * BS + threadIdx.x) * R;
     if(i >= S.size) return;
     if(S.use_u)
-        row_fwd_full_u<W - 1, W - 1, VT, XT, RT, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg,
+        row_fwd_full_u<W - 1, W - 1, VT, XT, RT, R>(S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg,
                                                     S.desc, (const RT*)S.r, (XT*)S.x, (XT*)S.t);
     else if(S.t)
-        row_fwd_full<VT, XT, RT, true, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
+        row_fwd_full<VT, XT, RT, true, R>(S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
                                           (XT*)S.t);
     else
-        row_fwd_full<VT, XT, RT, false, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
+        row_fwd_full<VT, XT, RT, false, R>(S.off + i, S.col, (const VT*)S.val, (const RT*)S.r, (XT*)S.x,
                                            nullptr);
 }
 
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(BS) void kb_bwd_full(const BSeg* __restrict__ segs,
     BSEG();
     const int64_t i = (lb * BS + threadIdx.x) * R;
     if(i >= S.size) return;
-    row_bwd_full<NS, VT, XT, R>(S.m, S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
+    row_bwd_full<NS, VT, XT, R>(S.off + i, S.col, (const VT*)S.val, (const VT*)S.dg, S.desc,
                                 (const XT*)S.t, (XT*)S.x);
 }
 
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(BS) void kb_restrict(const BSeg* __restrict__ segs,
     BSEG();
     const int64_t k = lb * BS + threadIdx.x;
     if(k >= S.size) return;
-    row_restrict<VT, XT, RT, XT>(S.m, k, S.col, (const VT*)S.val, S.f2c_fine, S.f2c_coarse, (const RT*)S.r,
+    row_restrict<VT, XT, RT, XT>(k, S.col, (const VT*)S.val, S.f2c_fine, S.f2c_coarse, (const RT*)S.r,
                                  (const XT*)S.x, (XT*)S.rc);
 }
 
@@ -482,7 +482,6 @@ struct Planner
             Problem* P = B.probs[k];
             const Level& L = P->L[l];
             if(c >= L.ncolors) return false;
-            s.m = L.m;
             s.off = L.offsets[c];
             s.size = L.sizes[c];
             s.col = L.ell_col;
@@ -516,7 +515,6 @@ struct Planner
         HPCGX_TRY(add(BK_RESTRICT, bs, false, r64, 0, 0.0, [&](int k, BSeg& s) {
             Problem* P = B.probs[k];
             const Level& L = P->L[l];
-            s.m = L.m;
             s.size = L.mc;
             s.col = L.ell_col;
             s.val = vals(L);
@@ -576,7 +574,6 @@ struct Planner
         return add(BK_SPMV, B.block[0], false, 1, dot ? 1 : 0, 0.0, [&](int k, BSeg& s) {
             Problem* P = B.probs[k];
             const Level& L = P->L[0];
-            s.m = L.m;
             s.off = 0;
             s.size = L.m;
             s.col = L.ell_col;

@@ -14,7 +14,6 @@ struct BSeg
 {
     int64_t blk0;      // first (logical) block of the segment in the launch
     int64_t nblk;      // blocks of the segment
-    int64_t m;         // rows of the problem's level (ELL stride)
     int64_t off;       // first row (or element) of the segment
     int64_t size;      // rows (or elements) in the segment
     const int32_t* col;
@@ -79,12 +78,12 @@ inline int64_t assign_blocks(BSeg* segs, size_t nseg, int block, bool vector_kin
 }
 
 // Rows per thread a sweep launch can use: 2 loads R consecutive rows of a
-// slot as one vector, which needs even offsets, sizes and ELL strides in
-// every segment; otherwise 1.
+// slot as one vector, which needs even offsets and sizes in every segment
+// (an even row's pair then shares its ELL slice); otherwise 1.
 inline int launch_rows(const BSeg* segs, size_t nseg, int rows)
 {
     for(size_t j = 0; j < nseg; ++j)
-        if((segs[j].off % 2) != 0 || (segs[j].size % 2) != 0 || (segs[j].m % 2) != 0) return 1;
+        if((segs[j].off % 2) != 0 || (segs[j].size % 2) != 0) return 1;
     return rows;
 }
 
@@ -147,7 +146,7 @@ int batch_create(Batch*& B, int nprob, const int* dims, int nlevels, bool with_f
 void batch_destroy(Batch* B);
 int batch_set_block(Batch& B, int level, int block);
 // rows per thread (1 or 2) of the sweep launches of a level; a launch whose
-// segments are not all even in offset, size and ELL stride runs with 1
+// segments are not all even in offset and size runs with 1
 int batch_set_rows(Batch& B, int level, int rows);
 int batch_cg(Batch& B, int mode, int max_iter, double tol, bool use_graph, int* niters, double* normr,
              double* normr0);

@@ -112,11 +112,11 @@ struct Level
     int offsets[MAX_COLORS + 1] = {};
     int round_sizes[MAX_COLORS] = {};   // per JPL round, as the reference records them
 
-    // column-major ELL: entry p of row r lives at p*m + r (64-bit offsets); the
+    // sliced ELL: entry p of row r lives at ell_at(p, r) (64-bit offsets); the
     // entries of a row in ascending column order, split-padded (kernels.hpp)
-    int32_t* ell_col = nullptr;   // [W*m], -1 padding between the diagonal and the right-aligned upper entries
-    double* ell_val = nullptr;    // [W*m] (f64 levels)
-    __half* ell_h = nullptr;      // [W*m] (mixed levels)
+    int32_t* ell_col = nullptr;   // [W*ell_rows(m)], -1 padding between the diagonal and the right-aligned upper entries
+    double* ell_val = nullptr;    // [W*ell_rows(m)] (f64 levels)
+    __half* ell_h = nullptr;      // [W*ell_rows(m)] (mixed levels)
     int32_t* diag_idx = nullptr;  // [m] slot of the diagonal
     int32_t* desc = nullptr;      // [m] row descriptor: diagonal slot, first last-colour slot, first upper slot
     double* inv_diag = nullptr;   // [m] 1/a_ii (f64)
@@ -180,6 +180,25 @@ __device__ __forceinline__ __half ntload(const __half* p)
 {
     return __ushort_as_half(__builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p)));
 }
+
+// Sliced ELL.  Rows are grouped in slices of ELL_SLICE consecutive rows;
+// slice s stores slot p of its rows contiguously at [(s*W + p)*ELL_SLICE,
+// +ELL_SLICE), so entry p of row r lives at ell_at(p, r).  A wave's slot loads
+// (64 or 128 rows of one slot) stay one contiguous segment, as in a
+// column-major ELL, but the 27 slots of a slice are neighbours in memory: a
+// wave streams one 5 KB (f16) to 41 KB (int32 + f64) block per array instead of
+// 27 segments 4*rows bytes apart, which HBM serves at a higher rate.  The
+// arrays hold ell_rows(m) = m rounded up to a whole slice; the rows past m are
+// padding (-1, 0).
+constexpr int64_t ELL_SLICE = 128;
+
+__host__ __device__ __forceinline__ int64_t ell_at(int p, int64_t r)
+{
+    const uint64_t u = (uint64_t)r;   // rows are non-negative: shifts, not divisions
+    return (int64_t)(((u / ELL_SLICE) * W + (uint64_t)p) * ELL_SLICE + (u % ELL_SLICE));
+}
+
+__host__ __device__ __forceinline__ int64_t ell_rows(int64_t m) { return (m + ELL_SLICE - 1) / ELL_SLICE * ELL_SLICE; }
 
 // Contiguous block ranges per XCD: the hardware hands workgroup i to XCD
 // i % 8, so logical block L = base(i % 8) + i / 8 gives each XCD one

@@ -246,15 +246,15 @@ struct Tiles
 
 // y = A x on R rows; returns the row sums in s[].
 template <typename VT, typename XT, int R>
-__device__ __forceinline__ void row_spmv(int64_t m, int64_t row, const int32_t* __restrict__ col,
+__device__ __forceinline__ void row_spmv(int64_t row, const int32_t* __restrict__ col,
                                          const VT* __restrict__ val, const XT* __restrict__ x, XT (&s)[R])
 {
     int32_t cc[W][R];
     VT av[W][R];
 #pragma unroll
-    for(int p = 0; p < W; ++p) ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
+    for(int p = 0; p < W; ++p) ld_nt<R>(col + ell_at(p, row), cc[p]);
 #pragma unroll
-    for(int p = 0; p < W; ++p) ld_nt<R>(val + (int64_t)p * m + row, av[p]);
+    for(int p = 0; p < W; ++p) ld_nt<R>(val + ell_at(p, row), av[p]);
     XT xv[W][R];
 #pragma unroll
     for(int p = 0; p < W; ++p)
@@ -298,15 +298,15 @@ __device__ __forceinline__ int desc_upper(int32_t v) { return (v >> 16) & 255; }
 constexpr int slot_dim(int n) { return n > 0 ? n : 1; }
 
 template <int NS, typename VT, int R>
-__device__ __forceinline__ void load_window(int64_t m, int64_t row, const int32_t* __restrict__ col,
+__device__ __forceinline__ void load_window(int64_t row, const int32_t* __restrict__ col,
                                             const VT* __restrict__ val, int start,
                                             int32_t (&cc)[slot_dim(NS)][R], VT (&av)[slot_dim(NS)][R])
 {
 #pragma unroll
     for(int q = 0; q < NS; ++q)
     {
-        ld_nt<R>(col + (int64_t)(start + q) * m + row, cc[q]);
-        ld_nt<R>(val + (int64_t)(start + q) * m + row, av[q]);
+        ld_nt<R>(col + ell_at(start + q, row), cc[q]);
+        ld_nt<R>(val + ell_at(start + q, row), av[q]);
     }
 }
 
@@ -342,7 +342,7 @@ __device__ __forceinline__ void sub_window(int start, const int (&lo)[R], const 
 // (earlier colours) are non-zero in x.  Window [0, NS), NS >= the colour's
 // largest d; the diagonal comes from its own array (dg).
 template <int NS, typename VT, typename XT, typename RT, int R>
-__device__ __forceinline__ void row_fwd_zero(int64_t m, int64_t row, const int32_t* __restrict__ col,
+__device__ __forceinline__ void row_fwd_zero(int64_t row, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, const VT* __restrict__ dg,
                                              const int32_t* __restrict__ desc, const RT* __restrict__ r,
                                              XT* __restrict__ x)
@@ -351,7 +351,7 @@ __device__ __forceinline__ void row_fwd_zero(int64_t m, int64_t row, const int32
     ld<R>(desc + row, ds);
     int32_t cc[slot_dim(NS)][R];
     VT av[slot_dim(NS)][R];
-    load_window<NS, VT, R>(m, row, col, val, 0, cc, av);
+    load_window<NS, VT, R>(row, col, val, 0, cc, av);
     XT d[R], s[R];
     ld_cvt<R>(dg + row, d);
     ld_cvt<R>(r + row, s);
@@ -378,7 +378,7 @@ __device__ __forceinline__ void row_fwd_zero(int64_t m, int64_t row, const int32
 // running sum (x*d minus its value on reaching slot tl), so it costs no second
 // accumulation.
 template <int NS, typename VT, typename XT, bool STORE_U, int R>
-__device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32_t* __restrict__ col,
+__device__ __forceinline__ void row_bwd_zero(int64_t row, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, const VT* __restrict__ dg,
                                              const int32_t* __restrict__ desc, XT* __restrict__ x,
                                              XT* __restrict__ u)
@@ -388,7 +388,7 @@ __device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32
     const int start = W - NS;
     int32_t cc[slot_dim(NS)][R];
     VT av[slot_dim(NS)][R];
-    load_window<NS, VT, R>(m, row, col, val, start, cc, av);
+    load_window<NS, VT, R>(row, col, val, start, cc, av);
     XT d[R], s[R];
     ld_cvt<R>(dg + row, d);
     ld<R>(x + row, s);
@@ -446,16 +446,16 @@ __device__ __forceinline__ void row_bwd_zero(int64_t m, int64_t row, const int32
 // t = r - L x (the sum up to the diagonal) for the backward sweep, whose lower
 // neighbours do not change in between.
 template <typename VT, typename XT, typename RT, bool STORE_T, int R>
-__device__ __forceinline__ void row_fwd_full(int64_t m, int64_t row, const int32_t* __restrict__ col,
+__device__ __forceinline__ void row_fwd_full(int64_t row, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, const RT* __restrict__ r,
                                              XT* __restrict__ x, XT* __restrict__ t)
 {
     int32_t cc[W][R];
     VT av[W][R];
 #pragma unroll
-    for(int p = 0; p < W; ++p) ld_nt<R>(col + (int64_t)p * m + row, cc[p]);
+    for(int p = 0; p < W; ++p) ld_nt<R>(col + ell_at(p, row), cc[p]);
 #pragma unroll
-    for(int p = 0; p < W; ++p) ld_nt<R>(val + (int64_t)p * m + row, av[p]);
+    for(int p = 0; p < W; ++p) ld_nt<R>(val + ell_at(p, row), av[p]);
     XT xv[W][R];
 #pragma unroll
     for(int p = 0; p < W; ++p)
@@ -502,7 +502,7 @@ __device__ __forceinline__ void row_fwd_full(int64_t m, int64_t row, const int32
 // diagonal and the last-colour slots only: windows [0, NA) and [W - NB, W);
 // t[row] becomes r - L x.
 template <int NA, int NB, typename VT, typename XT, typename RT, int R>
-__device__ __forceinline__ void row_fwd_full_u(int64_t m, int64_t row, const int32_t* __restrict__ col,
+__device__ __forceinline__ void row_fwd_full_u(int64_t row, const int32_t* __restrict__ col,
                                                const VT* __restrict__ val,
                                                const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                const RT* __restrict__ r, XT* __restrict__ x, XT* __restrict__ t)
@@ -512,8 +512,8 @@ __device__ __forceinline__ void row_fwd_full_u(int64_t m, int64_t row, const int
     const int bstart = W - NB;
     int32_t ca[slot_dim(NA)][R], cb[slot_dim(NB)][R];
     VT aa[slot_dim(NA)][R], ab[slot_dim(NB)][R];
-    load_window<NA, VT, R>(m, row, col, val, 0, ca, aa);
-    load_window<NB, VT, R>(m, row, col, val, bstart, cb, ab);
+    load_window<NA, VT, R>(row, col, val, 0, ca, aa);
+    load_window<NB, VT, R>(row, col, val, bstart, cb, ab);
     XT d[R], s[R], uu[R];
     ld_cvt<R>(dg + row, d);
     ld_cvt<R>(r + row, s);
@@ -548,7 +548,7 @@ __device__ __forceinline__ void row_fwd_full_u(int64_t m, int64_t row, const int
 // Full backward sweep from t: x = (t - U x) / d over the upper entries,
 // window [W - NS, W).
 template <int NS, typename VT, typename XT, int R>
-__device__ __forceinline__ void row_bwd_full(int64_t m, int64_t row, const int32_t* __restrict__ col,
+__device__ __forceinline__ void row_bwd_full(int64_t row, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, const VT* __restrict__ dg,
                                              const int32_t* __restrict__ desc, const XT* __restrict__ t,
                                              XT* __restrict__ x)
@@ -558,7 +558,7 @@ __device__ __forceinline__ void row_bwd_full(int64_t m, int64_t row, const int32
     const int start = W - NS;
     int32_t cc[slot_dim(NS)][R];
     VT av[slot_dim(NS)][R];
-    load_window<NS, VT, R>(m, row, col, val, start, cc, av);
+    load_window<NS, VT, R>(row, col, val, start, cc, av);
     XT d[R], s[R];
     ld_cvt<R>(dg + row, d);
     ld<R>(t + row, s);
@@ -580,7 +580,7 @@ __device__ __forceinline__ void row_bwd_full(int64_t m, int64_t row, const int32
 // Fused fine residual + injection for coarse natural row k:
 // rc[f2c_coarse[k]] = r[f] - sum_p a[f,p] x[col], f = f2c_fine[k].
 template <typename VT, typename XT, typename RT, typename CT>
-__device__ __forceinline__ void row_restrict(int64_t m, int64_t k, const int32_t* __restrict__ col,
+__device__ __forceinline__ void row_restrict(int64_t k, const int32_t* __restrict__ col,
                                              const VT* __restrict__ val, const int32_t* __restrict__ f2c_fine,
                                              const int32_t* __restrict__ f2c_coarse, const RT* __restrict__ r,
                                              const XT* __restrict__ x, CT* __restrict__ rc)
@@ -589,9 +589,9 @@ __device__ __forceinline__ void row_restrict(int64_t m, int64_t k, const int32_t
     int32_t cc[W][1];
     VT av[W][1];
 #pragma unroll
-    for(int p = 0; p < W; ++p) ld_nt<1>(col + (int64_t)p * m + row, cc[p]);
+    for(int p = 0; p < W; ++p) ld_nt<1>(col + ell_at(p, row), cc[p]);
 #pragma unroll
-    for(int p = 0; p < W; ++p) ld_nt<1>(val + (int64_t)p * m + row, av[p]);
+    for(int p = 0; p < W; ++p) ld_nt<1>(val + ell_at(p, row), av[p]);
     XT xv[W];
 #pragma unroll
     for(int p = 0; p < W; ++p) xv[p] = (cc[p][0] >= 0) ? x[cc[p][0]] : (XT)0;

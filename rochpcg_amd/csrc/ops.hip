@@ -1,9 +1,9 @@
 // ops.hip -- the solver's hot-path kernels for gfx950 (CDNA4, wave64).
 //
 // All operators are HBM-bound sparse/streaming work: one thread per row,
-// column-major ELL (entry p of row r at p*m+r) so every slot load of a wave is
-// one contiguous 256 B (cols) / 512 B (f64 values) segment, matrix streams
-// loaded non-temporally, x gathered through L2/MALL.  Row tiles are laid out
+// sliced ELL (entry p of row r at ell_at(p, r), common.hpp) so every slot load
+// of a wave is one contiguous 256 B (cols) / 512 B (f64 values) segment and
+// the slots of a slice follow each other, matrix streams loaded non-temporally, x gathered through L2/MALL.  Row tiles are laid out
 // so each XCD works on one contiguous spatial slab (xcd_block), which keeps
 // the neighbours a slab gathers inside that XCD's L2.
 //
@@ -27,7 +27,7 @@ namespace hpcgx
 {
 
 template <typename VT, typename XT, typename YT, bool DOT, int R, int BS>
-__global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restrict__ col,
+__global__ __launch_bounds__(BS) void k_spmv(const int32_t* __restrict__ col,
                                             const VT* __restrict__ val, const XT* __restrict__ x,
                                             YT* __restrict__ y, Tiles tl, RedArgs ra, const int* gate)
 {
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
         if(i >= tl.size[c]) continue;
         const int64_t row = tl.off[c] + i;
         XT s[R];
-        row_spmv<VT, XT, R>(m, row, col, val, x, s);
+        row_spmv<VT, XT, R>(row, col, val, x, s);
         st_cvt<R>(y + row, s);
         if(DOT)
         {
@@ -85,55 +85,55 @@ __global__ __launch_bounds__(BS) void k_spmv(int64_t m, const int32_t* __restric
     const int64_t row = off + i;
 
 template <int NS, typename VT, typename XT, typename RT, int R, int BS>
-__global__ __launch_bounds__(BS) void k_fwd_zero(int64_t m, int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col, const VT* __restrict__ val,
+__global__ __launch_bounds__(BS) void k_fwd_zero(int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                  const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                  const RT* __restrict__ r, XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    row_fwd_zero<NS, VT, XT, RT, R>(m, row, col, val, dg, desc, r, x);
+    row_fwd_zero<NS, VT, XT, RT, R>(row, col, val, dg, desc, r, x);
 }
 
 template <int NS, typename VT, typename XT, bool STORE_U, int R, int BS>
-__global__ __launch_bounds__(BS) void k_bwd_zero(int64_t m, int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col, const VT* __restrict__ val,
+__global__ __launch_bounds__(BS) void k_bwd_zero(int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                  const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                  XT* __restrict__ x, XT* __restrict__ u, const int* gate)
 {
     SWEEP_ROW();
-    row_bwd_zero<NS, VT, XT, STORE_U, R>(m, row, col, val, dg, desc, x, u);
+    row_bwd_zero<NS, VT, XT, STORE_U, R>(row, col, val, dg, desc, x, u);
 }
 
 template <typename VT, typename XT, typename RT, bool STORE_T, int R, int BS>
-__global__ __launch_bounds__(BS) void k_fwd_full(int64_t m, int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col,
+__global__ __launch_bounds__(BS) void k_fwd_full(int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const RT* __restrict__ r,
                                                  XT* __restrict__ x, XT* __restrict__ t, const int* gate)
 {
     SWEEP_ROW();
-    row_fwd_full<VT, XT, RT, STORE_T, R>(m, row, col, val, r, x, t);
+    row_fwd_full<VT, XT, RT, STORE_T, R>(row, col, val, r, x, t);
 }
 
 template <int NA, int NB, typename VT, typename XT, typename RT, int R, int BS>
-__global__ __launch_bounds__(BS) void k_fwd_full_u(int64_t m, int64_t off, int64_t size, ZMap zm,
+__global__ __launch_bounds__(BS) void k_fwd_full_u(int64_t off, int64_t size, ZMap zm,
                                                    const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                    const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                    const RT* __restrict__ r, XT* __restrict__ x, XT* __restrict__ t,
                                                    const int* gate)
 {
     SWEEP_ROW();
-    row_fwd_full_u<NA, NB, VT, XT, RT, R>(m, row, col, val, dg, desc, r, x, t);
+    row_fwd_full_u<NA, NB, VT, XT, RT, R>(row, col, val, dg, desc, r, x, t);
 }
 
 template <int NS, typename VT, typename XT, int R, int BS>
-__global__ __launch_bounds__(BS) void k_bwd_full(int64_t m, int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col, const VT* __restrict__ val,
+__global__ __launch_bounds__(BS) void k_bwd_full(int64_t off, int64_t size, ZMap zm, const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                  const VT* __restrict__ dg, const int32_t* __restrict__ desc,
                                                  const XT* __restrict__ t, XT* __restrict__ x, const int* gate)
 {
     SWEEP_ROW();
-    row_bwd_full<NS, VT, XT, R>(m, row, col, val, dg, desc, t, x);
+    row_bwd_full<NS, VT, XT, R>(row, col, val, dg, desc, t, x);
 }
 #undef SWEEP_ROW
 
 template <typename VT, typename XT, typename RT, typename CT, int BS>
-__global__ __launch_bounds__(BS) void k_restrict(int64_t m, int64_t mc, const int32_t* __restrict__ col,
+__global__ __launch_bounds__(BS) void k_restrict(int64_t mc, const int32_t* __restrict__ col,
                                                  const VT* __restrict__ val, const int32_t* __restrict__ f2c_fine,
                                                  const int32_t* __restrict__ f2c_coarse, const RT* __restrict__ r,
                                                  const XT* __restrict__ x, CT* __restrict__ rc, const int* gate)
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(BS) void k_restrict(int64_t m, int64_t mc, const in
     if(gated(gate)) return;
     const int64_t k = xcd_block(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     if(k >= mc) return;
-    row_restrict<VT, XT, RT, CT>(m, k, col, val, f2c_fine, f2c_coarse, r, x, rc);
+    row_restrict<VT, XT, RT, CT>(k, col, val, f2c_fine, f2c_coarse, r, x, rc);
 }
 
 // Prolongation: xf[f2c_fine[k]] += xc[f2c_coarse[k]].
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(BS) void k_prolong(int64_t mc, const int32_t* __res
 // Reference-mode pieces (ComputeSPMV.cpp:87-128 on MGData::Axf and
 // ComputeRestriction.cpp:59-79): A x at the injected rows only, and
 // rc = rf - Axf at those rows.
-__global__ __launch_bounds__(256) void k_spmv_rows(int64_t m, int64_t mc, const int32_t* __restrict__ col,
+__global__ __launch_bounds__(256) void k_spmv_rows(int64_t mc, const int32_t* __restrict__ col,
                                                    const double* __restrict__ val, const int32_t* __restrict__ rows,
                                                    const double* __restrict__ x, double* __restrict__ y)
 {
@@ -170,8 +170,8 @@ __global__ __launch_bounds__(256) void k_spmv_rows(int64_t m, int64_t mc, const 
     double s = 0.0;
     for(int p = 0; p < W; ++p)
     {
-        const int32_t c = col[(int64_t)p * m + row];
-        if(c >= 0) s = fmaT(val[(int64_t)p * m + row], x[c], s);
+        const int32_t c = col[ell_at(p, row)];
+        if(c >= 0) s = fmaT(val[ell_at(p, row)], x[c], s);
     }
     y[row] = s;
 }
@@ -381,7 +381,7 @@ template <typename VT>
 __global__ void k_get_diag(int64_t m, const int32_t* __restrict__ dgi, const VT* __restrict__ val, double* __restrict__ d)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if(r < m) d[r] = cvt<double>(val[(int64_t)dgi[r] * m + r]);
+    if(r < m) d[r] = cvt<double>(val[ell_at(dgi[r], r)]);
 }
 
 __global__ void k_set_diag(int64_t m, const int32_t* __restrict__ dgi, double* __restrict__ val,
@@ -390,7 +390,7 @@ __global__ void k_set_diag(int64_t m, const int32_t* __restrict__ dgi, double* _
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(r < m)
     {
-        val[(int64_t)dgi[r] * m + r] = d[r];
+        val[ell_at(dgi[r], r)] = d[r];
         dg[r] = d[r];
         if(inv_diag) inv_diag[r] = 1.0 / d[r];
     }
@@ -445,10 +445,11 @@ bool shape_supported(int rows, int block)
     return (rows == 1 || rows == 2) && (block == 64 || block == 128 || block == 256);
 }
 
-// R rows per thread need every colour block (and m) to be R-aligned.
+// R rows per thread need every colour block to be R-aligned (a thread's R
+// rows then share their ELL slice).
 static Shape fit(const Level& L, Shape sh)
 {
-    bool ok = (L.m % sh.rows) == 0;
+    bool ok = true;
     for(int c = 0; c < L.ncolors && ok; ++c) ok = (L.offsets[c] % sh.rows) == 0 && (L.sizes[c] % sh.rows) == 0;
     if(!ok) sh.rows = 1;
     return sh;
@@ -522,13 +523,13 @@ static int spmv_t(const Level& L, const void* x, void* y, const RedArgs* ra, hip
             static int64_t resident = 0;
             Tiles tl = make_tiles(L, BS * R, true, resident_blocks(k_spmv<VT, XT, YT, true, R, BS>, BS, resident),
                                   cap, nb);
-            hipLaunchKernelGGL((k_spmv<VT, XT, YT, true, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m, L.ell_col,
+            hipLaunchKernelGGL((k_spmv<VT, XT, YT, true, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.ell_col,
                                vals_of<VT>(L), (const XT*)x, (YT*)y, tl, *ra, gate);
             HPCGX_KCHECK();
             return launch_fold(nb, *ra, s, gate);
         }
         Tiles tl = make_tiles(L, BS * R, false, 0, 0, nb);
-        hipLaunchKernelGGL((k_spmv<VT, XT, YT, false, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.m, L.ell_col,
+        hipLaunchKernelGGL((k_spmv<VT, XT, YT, false, R, BS>), dim3((unsigned)nb), dim3(BS), 0, s, L.ell_col,
                            vals_of<VT>(L), (const XT*)x, (YT*)y, tl, RedArgs{}, gate);
         HPCGX_KCHECK();
         return 0;
@@ -627,7 +628,7 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
             const int n = L.win[WIN_LOWER][c];
             with_window(n, [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
-                hipLaunchKernelGGL((k_fwd_zero<NS, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
+                hipLaunchKernelGGL((k_fwd_zero<NS, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s,
                                    (int64_t)L.offsets[c], sz, zm, L.ell_col, val, dg, L.desc, r, x, gate);
             });
         });
@@ -640,10 +641,10 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
             with_window(n, [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
                 if(part == 4)
-                    hipLaunchKernelGGL((k_bwd_zero<NS, VT, XT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
+                    hipLaunchKernelGGL((k_bwd_zero<NS, VT, XT, true, R, BS>), grid, dim3(BS), 0, s,
                                        (int64_t)L.offsets[c], sz, zm, L.ell_col, val, dg, L.desc, x, t, gate);
                 else
-                    hipLaunchKernelGGL((k_bwd_zero<NS, VT, XT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
+                    hipLaunchKernelGGL((k_bwd_zero<NS, VT, XT, false, R, BS>), grid, dim3(BS), 0, s,
                                        (int64_t)L.offsets[c], sz, zm, L.ell_col, val, dg, L.desc, x, t, gate);
             });
         });
@@ -656,16 +657,16 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
                 const int na = L.win[WIN_LOWER][c], nb = L.win[WIN_TAIL][c];
                 with_window_pair(na, nb, [&](auto NA_, auto NB_) {
                     constexpr int NA = decltype(NA_)::value, NB = decltype(NB_)::value;
-                    hipLaunchKernelGGL((k_fwd_full_u<NA, NB, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s, L.m,
+                    hipLaunchKernelGGL((k_fwd_full_u<NA, NB, VT, XT, RT, R, BS>), grid, dim3(BS), 0, s,
                                        (int64_t)L.offsets[c], sz, zm, L.ell_col, val, dg, L.desc, r, x, t,
                                        gate);
                 });
             }
             else if(c < last)
-                hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, true, R, BS>), grid, dim3(BS), 0, s, L.m,
+                hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, true, R, BS>), grid, dim3(BS), 0, s,
                                    (int64_t)L.offsets[c], sz, zm, L.ell_col, val, r, x, t, gate);
             else
-                hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, false, R, BS>), grid, dim3(BS), 0, s, L.m,
+                hipLaunchKernelGGL((k_fwd_full<VT, XT, RT, false, R, BS>), grid, dim3(BS), 0, s,
                                    (int64_t)L.offsets[c], sz, zm, L.ell_col, val, r, x, t, gate);
         });
     case 3:
@@ -675,7 +676,7 @@ static int sweep_t(const Level& L, int part, const void* rv, void* xv, void* tv,
             const int n = L.win[WIN_UPPER][c];
             with_window(n, [&](auto NN) {
                 constexpr int NS = decltype(NN)::value;
-                hipLaunchKernelGGL((k_bwd_full<NS, VT, XT, R, BS>), grid, dim3(BS), 0, s, L.m,
+                hipLaunchKernelGGL((k_bwd_full<NS, VT, XT, R, BS>), grid, dim3(BS), 0, s,
                                    (int64_t)L.offsets[c], sz, zm, L.ell_col, val, dg, L.desc, (const XT*)t, x, gate);
             });
         });
@@ -727,7 +728,7 @@ static int restrict_t(const Level& F, const void* r, const void* x, void* rc, hi
     const int bs = F.shp[std::is_same<VT, double>::value ? PREC_F64 : PREC_MIXED][SH_RESTRICT].block;
     auto go = [&](auto BB) -> int {
         constexpr int BS = decltype(BB)::value;
-        hipLaunchKernelGGL((k_restrict<VT, XT, RT, XT, BS>), dim3(nblk(F.mc, BS)), dim3(BS), 0, s, F.m, F.mc,
+        hipLaunchKernelGGL((k_restrict<VT, XT, RT, XT, BS>), dim3(nblk(F.mc, BS)), dim3(BS), 0, s, F.mc,
                            F.ell_col, vals_of<VT>(F), F.f2c_fine, F.f2c_coarse, (const RT*)r, (const XT*)x,
                            (XT*)rc, gate);
         HPCGX_KCHECK();
@@ -766,7 +767,7 @@ int launch_spmv_injected(const Level& F, const double* x, double* y, hipStream_t
         set_error("injected-row SpMV needs an f64 level with a coarser level");
         return -2;
     }
-    hipLaunchKernelGGL(k_spmv_rows, dim3(nblk(F.mc, 256)), dim3(256), 0, s, F.m, F.mc, F.ell_col, F.ell_val, F.f2c_fine,
+    hipLaunchKernelGGL(k_spmv_rows, dim3(nblk(F.mc, 256)), dim3(256), 0, s, F.mc, F.ell_col, F.ell_val, F.f2c_fine,
                        x, y);
     HPCGX_KCHECK();
     return 0;

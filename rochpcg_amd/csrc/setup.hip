@@ -187,7 +187,18 @@ __global__ __launch_bounds__(256) void k_build_ell(Grid g, int64_t m, const int3
                                                    VT* __restrict__ dvals)
 {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if(r >= m) return;
+    if(r >= ell_rows(m)) return;
+    if(r >= m)
+    {
+        // padding rows of the last slice
+#pragma unroll
+        for(int p = 0; p < W; ++p)
+        {
+            ell_col[ell_at(p, r)] = -1;
+            ell_val[ell_at(p, r)] = (VT)0.0f;
+        }
+        return;
+    }
     const int64_t i = iperm[r];
     int ix, iy, iz;
     g.coords(i, ix, iy, iz);
@@ -237,8 +248,8 @@ __global__ __launch_bounds__(256) void k_build_ell(Grid g, int64_t m, const int3
     {
         const int c = key[p];
         const bool valid = (c != INT_MAX);
-        ell_col[(int64_t)p * m + r] = valid ? c : -1;
-        ell_val[(int64_t)p * m + r] = (VT)(valid ? (p == dg ? 26.0f : -1.0f) : 0.0f);
+        ell_col[ell_at(p, r)] = valid ? c : -1;
+        ell_val[ell_at(p, r)] = (VT)(valid ? (p == dg ? 26.0f : -1.0f) : 0.0f);
     }
     diag_idx[r] = dg;
     desc[r] = dg | ((tl0 + shift) << 8) | ((dg + 1 + shift) << 16);
@@ -549,25 +560,25 @@ int build_level(Level& L, int nx, int ny, int nz, bool with_f64, bool with_f16, 
     HPCGX_TRY(color_level(L, s));
     const int64_t m = L.m;
     Grid g{nx, ny, nz};
-    HPCGX_HIP(hipMalloc(&L.ell_col, sizeof(int32_t) * W * m));
+    HPCGX_HIP(hipMalloc(&L.ell_col, sizeof(int32_t) * W * ell_rows(m)));
     HPCGX_HIP(hipMalloc(&L.diag_idx, sizeof(int32_t) * m));
     HPCGX_HIP(hipMalloc(&L.desc, sizeof(int32_t) * m));
     const int32_t last_off = L.offsets[L.ncolors - 1];
     if(with_f64)
     {
-        HPCGX_HIP(hipMalloc(&L.ell_val, sizeof(double) * W * m));
+        HPCGX_HIP(hipMalloc(&L.ell_val, sizeof(double) * W * ell_rows(m)));
         HPCGX_HIP(hipMalloc(&L.inv_diag, sizeof(double) * m));
         HPCGX_HIP(hipMalloc(&L.diag64, sizeof(double) * m));
-        hipLaunchKernelGGL(k_build_ell<double>, dim3(grid_of(m, 256)), dim3(256), 0, s, g, m, L.perm,
+        hipLaunchKernelGGL(k_build_ell<double>, dim3(grid_of(ell_rows(m), 256)), dim3(256), 0, s, g, m, L.perm,
                            L.iperm_tmp, L.ell_col, L.ell_val, L.diag_idx, L.desc, last_off, L.inv_diag,
                            L.diag64);
         HPCGX_HIP(hipGetLastError());
     }
     if(with_f16)
     {
-        HPCGX_HIP(hipMalloc(&L.ell_h, sizeof(__half) * W * m));
+        HPCGX_HIP(hipMalloc(&L.ell_h, sizeof(__half) * W * ell_rows(m)));
         HPCGX_HIP(hipMalloc(&L.diag_h, sizeof(__half) * m));
-        hipLaunchKernelGGL(k_build_ell<__half>, dim3(grid_of(m, 256)), dim3(256), 0, s, g, m, L.perm,
+        hipLaunchKernelGGL(k_build_ell<__half>, dim3(grid_of(ell_rows(m), 256)), dim3(256), 0, s, g, m, L.perm,
                            L.iperm_tmp, L.ell_col, L.ell_h, L.diag_idx, L.desc, last_off, (double*)nullptr,
                            L.diag_h);
         HPCGX_HIP(hipGetLastError());

@@ -552,10 +552,10 @@ int64_t problem_device_bytes(const Problem& P)
     for(int l = 0; l < P.nlevels; ++l)
     {
         const Level& L = P.L[l];
-        const int64_t m = L.m;
-        b += 4 * W * m + 4 * 4 * m;   // columns; diag_idx, desc, perm, colors
-        if(P.has_f64) b += 8 * W * m + 8 * m + 8 * m + (l > 0 ? 16 * m : 0);   // values, 1/a_ii, t, r, x
-        if(P.has_mixed) b += 2 * W * m + 4 * m + 4 * m + (l > 0 ? 8 * m : 0);
+        const int64_t m = L.m, me = ell_rows(m);
+        b += 4 * W * me + 4 * 4 * m;   // columns; diag_idx, desc, perm, colors
+        if(P.has_f64) b += 8 * W * me + 8 * m + 8 * m + (l > 0 ? 16 * m : 0);   // values, 1/a_ii, t, r, x
+        if(P.has_mixed) b += 2 * W * me + 4 * m + 4 * m + (l > 0 ? 8 * m : 0);
         b += 3 * 4 * L.mc;   // transfer maps
     }
     const int64_t m0 = P.L[0].m;

@@ -5,13 +5,12 @@
 // header is HIP code.
 //
 // stdin: per launch a line "block vector_kind rows nseg" followed by nseg
-// lines "off size m".  For every launch it checks that
+// lines "off size".  For every launch it checks that
 //   - find_seg maps every logical block to the segment holding it;
 //   - xcd_block is a permutation of the launch's blocks;
 //   - every row (element) of every segment is visited by exactly one thread
 //     (R consecutive rows per thread in row launches, chunks in vector ones);
-//   - two rows per thread are used only where every offset, size and stride
-//     is even.
+//   - two rows per thread are used only where every offset and size is even.
 // Prints "ok <launches> <blocks>" or the first failure.
 #include <cstdio>
 #include <vector>
@@ -29,16 +28,15 @@ int main()
         std::vector<BSeg> segs((size_t)nseg);
         for(BSeg& s : segs)
         {
-            long long off, size, m;
-            if(scanf("%lld %lld %lld", &off, &size, &m) != 3) return 2;
+            long long off, size;
+            if(scanf("%lld %lld", &off, &size) != 2) return 2;
             s = BSeg{};
             s.off = off;
             s.size = size;
-            s.m = m;
         }
         const int rows = vector_kind ? 1 : launch_rows(segs.data(), segs.size(), rows_req);
         bool even = true;
-        for(const BSeg& s : segs) even = even && s.off % 2 == 0 && s.size % 2 == 0 && s.m % 2 == 0;
+        for(const BSeg& s : segs) even = even && s.off % 2 == 0 && s.size % 2 == 0;
         if(rows == 2 && !even)
         {
             printf("FAIL launch %ld: two rows per thread on an odd segment\n", launches);

@@ -37,19 +37,19 @@ def _colour_tables(oracle_mod, dims, nlevels=4):
 
 
 def _launches(tables, block, rows):
-    """(block, vector_kind, rows, [(off, size, m)]) of every launch of the
+    """(block, vector_kind, rows, [(off, size)]) of every launch of the
     preamble and one iteration, in the planner's order."""
     L = []
     m0 = [t[0][2] for t in tables]
 
     def vec():
-        L.append((256, 1, 1, [(0, m, 0) for m in m0]))
+        L.append((256, 1, 1, [(0, m) for m in m0]))
 
     def row(blk, r, segs):
         if segs:
             L.append((blk, 0, r, segs))
     vec()                                          # p = x
-    row(block[0], 1, [(0, m, m) for m in m0])      # Ap = A p
+    row(block[0], 1, [(0, m) for m in m0])         # Ap = A p
     vec()                                          # r = b - Ap
     vec()                                          # ||r||
     nlev = len(tables[0])
@@ -59,17 +59,17 @@ def _launches(tables, block, rows):
         last = lv + 1 == nlev
 
         def seg(t, c):
-            sizes, offsets, m, _ = t[lv]
-            return (int(offsets[c]), int(sizes[c]), m) if c < len(sizes) else None
+            sizes, offsets, _, _ = t[lv]
+            return (int(offsets[c]), int(sizes[c])) if c < len(sizes) else None
         for c in range(C):
             row(block[lv], rows[lv], [s for s in (seg(t, c) for t in tables) if s])
         for c in range(C - 2, -1, -1):
             row(block[lv], rows[lv], [seg(t, c) for t in tables if c <= len(t[lv][0]) - 2])
         if last:
             return
-        row(block[lv], 1, [(0, t[lv][3], t[lv][2]) for t in tables])   # restriction
+        row(block[lv], 1, [(0, t[lv][3]) for t in tables])   # restriction
         mg(lv + 1)
-        row(block[lv], 1, [(0, t[lv][3], 0) for t in tables])          # prolongation
+        row(block[lv], 1, [(0, t[lv][3]) for t in tables])   # prolongation
         for c in range(C):
             row(block[lv], rows[lv], [s for s in (seg(t, c) for t in tables) if s])
         for c in range(C - 2, -1, -1):
@@ -77,7 +77,7 @@ def _launches(tables, block, rows):
     mg(0)
     vec()                                          # r'z
     vec()                                          # p update
-    row(block[0], 1, [(0, m, m) for m in m0])      # Ap = A p (+ p'Ap)
+    row(block[0], 1, [(0, m) for m in m0])         # Ap = A p (+ p'Ap)
     vec()                                          # r, x update and ||r||
     return L
 
@@ -101,11 +101,11 @@ def test_batch_plan_64_problems(oracle_mod, checker):
         for rows in (1, 2):
             for blk, vk, r, segs in _launches(tables, [block] * 4, [rows] * 4):
                 lines.append(f"{blk} {vk} {r} {len(segs)}")
-                lines += [f"{o} {s} {m}" for o, s, m in segs]
+                lines += [f"{o} {s}" for o, s in segs]
     # the shape the tuner settled on for this workload: 128 threads, two rows on the coarsest level
     for blk, vk, r, segs in _launches(tables, [128] * 4, [1, 1, 1, 2]):
         lines.append(f"{blk} {vk} {r} {len(segs)}")
-        lines += [f"{o} {s} {m}" for o, s, m in segs]
+        lines += [f"{o} {s}" for o, s in segs]
     out = subprocess.run([checker], input="\n".join(lines) + "\n", capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.startswith("ok ")
@@ -116,9 +116,9 @@ def test_batch_plan_64_problems(oracle_mod, checker):
 def test_batch_plan_odd_segments(checker):
     """Segments of odd offset or size force one row per thread; single-row
     and single-element segments, and the reduction's block cap."""
-    lines = ["256 0 2 3", "0 7 7", "7 2 9", "9 1 9",
-             "64 0 2 2", "0 4096 8192", "4096 4096 8192",
-             "256 1 1 2", "0 5000000 0", "0 1 0"]
+    lines = ["256 0 2 3", "0 7", "7 2", "9 1",
+             "64 0 2 2", "0 4096", "4096 4096",
+             "256 1 1 2", "0 5000000", "0 1"]
     out = subprocess.run([checker], input="\n".join(lines) + "\n", capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.split()[:2] == ["ok", "3"]

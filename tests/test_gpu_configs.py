@@ -9,7 +9,7 @@
   64 uneven        nx, ny, nz each from {16, 24, ..., 96} (splitmix64 seed 7),
                    problem k with x* from seed 7 + k, FP64 and mixed to 1e-9.
   512^3            closed-form colours, permutation and ELL rows whose 64-bit
-                   offsets p*rows + r exceed 2^31; x* from seed 99, FP64 and
+                   offsets exceed 2^31; x* from seed 99, FP64 and
                    mixed (true residual every 10 iterations) to 1e-9.
 
 The residuals and errors of the full-size solves are computed on the host by
@@ -182,8 +182,8 @@ def test_config_batched_64_history_vs_oracle(hip, oracle_mod, batch64):
 def test_config_512(hip, oracle_mod):
     """512^3 (134M rows, 27 * rows > 2^31): the level-0 colours and
     permutation against the closed form, the ELL rows of windows in every
-    colour (every slot p >= 16 sits at p * rows + r >= 2^31) against the
-    oracle's split layout, then the FP64 and the mixed solve (seed 99, true
+    colour (those past row 79.6M sit wholly beyond element 2^31 of the sliced
+    arrays) against the oracle's split layout, then the FP64 and the mixed solve (seed 99, true
     residual every 10 iterations) to 1e-9."""
     n = 512
     dims = (n, n, n)
@@ -206,16 +206,15 @@ def test_config_512(hip, oracle_mod):
     _say("config 512^3: ELL rows at 64-bit offsets")
     win = 1024
     starts = [int(c * (m // 8) + (c * 7919) % (m // 8 - win)) for c in range(8)] + [m - win]
+    # the sliced ELL puts entry p of row r at ((r//128)*27 + p)*128 + r%128 >= 27*(r - 127):
+    # the windows of colours 5-7 and the last one lie wholly past element 2^31
+    assert sum(27 * (r0 - 127) >= 2 ** 31 for r0 in starts) >= 4
     for r0 in starts:
         rows = np.arange(r0, r0 + win)
         cols_e, vals_e = oracle_mod.split_ell_rows(*dims, perm, iperm, rows)
-        for p in range(27):
-            cols = V["ell_col"].host(p * m + r0, win)
-            assert np.array_equal(cols, cols_e[:, p]), (r0, p)
-            vals = V["ell_val"].host(p * m + r0, win)
-            assert np.array_equal(vals, vals_e[:, p]), (r0, p)
-            if p >= 16:
-                assert np.array_equal(V["ell_f16"].host(p * m + r0, win).astype(np.float64), vals_e[:, p])
+        assert np.array_equal(hip.ell_rows(V["ell_col"], r0, win), cols_e), r0
+        assert np.array_equal(hip.ell_rows(V["ell_val"], r0, win), vals_e), r0
+        assert np.array_equal(hip.ell_rows(V["ell_f16"], r0, win).astype(np.float64), vals_e), r0
         lower = ((cols_e >= 0) & (cols_e < rows[:, None])).sum(axis=1)
         assert np.array_equal(V["diag_idx"].host(r0, win), lower)
         assert np.array_equal(V["desc"].host(r0, win) & 255, lower)
