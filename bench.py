@@ -284,6 +284,16 @@ def main():
     mg_ms = ph["mg"] / K
     mg_gbps = mg64 / (mg_ms * 1e-3) / 1e9
     log(f"phases (ms per iteration): " + ", ".join(f"{k} {v / K:.3f}" for k, v in ph.items()))
+    # the box's read ceiling for the matrix stream the SpMV reads (the level-0
+    # column and value arrays, 16-byte loads, no gathers), in the same run, and
+    # the mixed-mode outer SpMV (f16 values, f64 vectors) timed as the CG runs it
+    me = hip.ell_padded_rows(G.m)
+    ceil64 = 27 * 12 * me / (G.time_operator(hip.OP_READ_STREAM, 0, mixed=False, reps=10) * 1e-3) / 1e9
+    ceil16 = 27 * 6 * me / (G.time_operator(hip.OP_READ_STREAM, 0, mixed=True, reps=10) * 1e-3) / 1e9
+    spmv16_ms = G.time_operator(hip.OP_SPMV, 0, mixed=True, reps=K)
+    spmv16_gbps = spmv_bytes(G.m, 2) / (spmv16_ms * 1e-3) / 1e9
+    log(f"read ceiling of the matrix stream: {ceil64:.0f} GB/s (f64 values), {ceil16:.0f} GB/s (f16 values); "
+        f"mixed SpMV {spmv16_ms:.3f} ms, {spmv16_gbps:.0f} GB/s")
 
     out = {
         "metric": METRIC,
@@ -318,7 +328,14 @@ def main():
                      "bound": "hbm", "achieved": round(spmv_gbps, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(spmv_gbps / HBM_PEAK_GBPS, 4),
                      "traffic": profiled_traffic(dims), "bytes_per_launch": spmv_bytes(G.m),
-                     "avg_launch_ms": round(spmv_ms, 4)},
+                     "avg_launch_ms": round(spmv_ms, 4),
+                     "read_ceiling_GBps": round(ceil64, 1), "frac_of_read_ceiling": round(spmv_gbps / ceil64, 4)},
+        "mixed_spmv_roofline": {"kernel": "k_spmv (f16 values, f64 vectors, fused p'Ap), level 0",
+                                "bound": "hbm", "achieved": round(spmv16_gbps, 1), "peak": HBM_PEAK_GBPS,
+                                "unit": "GB/s", "frac": round(spmv16_gbps / HBM_PEAK_GBPS, 4),
+                                "bytes_per_launch": spmv_bytes(G.m, 2), "avg_launch_ms": round(spmv16_ms, 4),
+                                "read_ceiling_GBps": round(ceil16, 1),
+                                "frac_of_read_ceiling": round(spmv16_gbps / ceil16, 4)},
         "mg_roofline": {"kernels": "FP64 V-cycle, all levels (sweeps, restriction, prolongation)",
                         "bound": "hbm", "achieved": round(mg_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                         "frac": round(mg_gbps / HBM_PEAK_GBPS, 4), "bytes_per_vcycle": int(mg64),

@@ -254,9 +254,11 @@ int hpcgx_batch_history(hpcgx_batch* b, int k, double* history, int hist_len, in
  * op 0 = SpMV, 1 = SymGS, 2 = zero-guess SymGS, 3 = MG from `level`,
  * 4/5 = zero-guess forward/backward sweeps, 6/7 = forward/backward sweeps
  * (in the form the V-cycle runs them on that level), 8 = fused residual +
- * restriction; prec 0 = f64 store, 1 = mixed store (SpMV: f16 values with
- * f64 vectors).  SpMV on level 0 is timed as the CG runs it: with the fused
- * p'Ap partials and their fold. */
+ * restriction, 9 = one read of the level's ELL column and value arrays with
+ * 16-byte loads and no gathers (the read ceiling of the matrix stream that
+ * the SpMV reads: 27 * (4 + value bytes) per row); prec 0 = f64 store,
+ * 1 = mixed store (SpMV: f16 values with f64 vectors).  SpMV on level 0 is
+ * timed as the CG runs it: with the fused p'Ap partials and their fold. */
 int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps, double* avg_ms);
 /* launch shape of an operator on a level (the autotuner's knobs; no
  * reference counterpart -- the reference hard-codes 1024/256/128-thread

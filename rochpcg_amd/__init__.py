@@ -27,7 +27,7 @@ CG_MIXED = 1
 
 # hpcgx_time_operator op codes
 (OP_SPMV, OP_SYMGS, OP_SYMGS_ZERO, OP_MG, OP_FWD_ZERO, OP_BWD_ZERO, OP_FWD_FULL, OP_BWD_FULL,
- OP_RESTRICT) = range(9)
+ OP_RESTRICT, OP_READ_STREAM) = range(10)
 PHASES = ("spmv", "mg", "ddot", "waxpby", "true_residual")
 # launch-shape operators (hpcgx_set_shape)
 (SH_SPMV, SH_FWD_ZERO, SH_BWD_ZERO, SH_FWD_FULL, SH_BWD_FULL, SH_RESTRICT,

@@ -481,7 +481,7 @@ int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps,
     Problem* P = p->P;
     const Level& L = P->L[level];
     const bool mixed = prec != 0;
-    if((mixed && !P->has_mixed) || (!mixed && !P->has_f64) || op < 0 || op > 8 || reps < 1)
+    if((mixed && !P->has_mixed) || (!mixed && !P->has_f64) || op < 0 || op > 9 || reps < 1)
     {
         set_error("operator not available for this problem");
         return -2;
@@ -521,6 +521,7 @@ int hpcgx_time_operator(hpcgx_problem* p, int op, int level, int prec, int reps,
             return launch_sweep(L, pr, part, r, r_f64, x, V[level].t, P->s, nullptr);
         }
         case 8: return launch_restrict(L, pr, r, r_f64, x, V[level + 1].r, P->s, nullptr);
+        case 9: return launch_read_stream(L, pr, P->partial, P->partial_cap, P->s);
         }
         return -2;
     };

@@ -781,6 +781,55 @@ int launch_restrict_diff(const Level& F, const double* rf, const double* Axf, do
     return 0;
 }
 
+// The read ceiling of a level's matrix stream: every byte of its column and
+// value arrays read once with 16-byte non-temporal loads, four in flight per
+// thread, and no gathers -- what the SpMV streams, without the rest of its
+// work.  The xor of the data decides a store that never happens in practice,
+// so the loads cannot be dropped.
+typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_read_stream(int64_t na, const U32x4* __restrict__ a, int64_t nb,
+                                                    const U32x4* __restrict__ b, double* __restrict__ sink)
+{
+    const int64_t stride = (int64_t)gridDim.x * BS;
+    U32x4 acc = {0u, 0u, 0u, 0u};
+    auto pass = [&](int64_t n, const U32x4* __restrict__ v) {
+        int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
+        for(; i + 3 * stride < n; i += 4 * stride)
+        {
+            const U32x4 w0 = __builtin_nontemporal_load(v + i);
+            const U32x4 w1 = __builtin_nontemporal_load(v + i + stride);
+            const U32x4 w2 = __builtin_nontemporal_load(v + i + 2 * stride);
+            const U32x4 w3 = __builtin_nontemporal_load(v + i + 3 * stride);
+            acc ^= (w0 ^ w1) ^ (w2 ^ w3);
+        }
+        for(; i < n; i += stride) acc ^= __builtin_nontemporal_load(v + i);
+    };
+    pass(na, a);
+    pass(nb, b);
+    if((acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x9e3779b9u) sink[blockIdx.x] = 1.0;
+}
+
+int launch_read_stream(const Level& L, Prec prec, double* sink, int64_t sink_cap, hipStream_t s)
+{
+    const int64_t me = ell_rows(L.m);
+    const void* val = prec == PREC_F64 ? (const void*)L.ell_val : (const void*)L.ell_h;
+    const int64_t vbytes = W * me * (prec == PREC_F64 ? 8 : 2);
+    const int64_t cbytes = W * me * 4;
+    if(!val || (vbytes % 16) || (cbytes % 16))
+    {
+        set_error("read stream needs the level's value store and 16-byte multiples");
+        return -2;
+    }
+    constexpr int BS = 256;
+    const unsigned g = (unsigned)std::min<int64_t>(RED_BLOCKS, sink_cap);
+    hipLaunchKernelGGL((k_read_stream<BS>), dim3(g), dim3(BS), 0, s, cbytes / 16, (const U32x4*)L.ell_col,
+                       vbytes / 16, (const U32x4*)val, sink);
+    HPCGX_KCHECK();
+    return 0;
+}
+
 static unsigned red_grid(int64_t n)
 {
     int64_t g = ceil_div(n, (int64_t)RED_THREADS * 4);

@@ -58,6 +58,10 @@ int launch_restrict(const Level& F, Prec prec, const void* r, bool r_f64, const 
 int launch_prolong(const Level& F, Prec prec, const void* xc, void* x, hipStream_t s, const int* gate);
 int launch_spmv_injected(const Level& F, const double* x, double* y, hipStream_t s);
 int launch_restrict_diff(const Level& F, const double* rf, const double* Axf, double* rc, hipStream_t s);
+// one read of a level's column and value arrays (16-byte non-temporal loads,
+// no gathers): the read ceiling of the matrix stream; sink holds >= min(4096,
+// sink_cap) doubles and is not normally written
+int launch_read_stream(const Level& L, Prec prec, double* sink, int64_t sink_cap, hipStream_t s);
 // blocks (= partial slots) of a streaming vector reduction over n elements:
 // at most RED_BLOCKS whatever n
 int64_t reduction_blocks(int64_t n);
