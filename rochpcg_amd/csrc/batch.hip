@@ -4,8 +4,9 @@
 // one launch per (operator, level, colour) covering every problem, so a whole
 // batched iteration is ~115 launches whatever the number of problems, and is
 // captured once as a hipGraph.  Blocks map to (problem, row block) through a
-// segment table in device memory; a problem's segments return at once after
-// its own convergence flag is set.
+// segment table in device memory (the segment records, then the segment of
+// every block); a problem's segments return at once after its own
+// convergence flag is set.
 //
 // Per-row arithmetic is shared with the single-problem kernels
 // (kernels.hpp), so each problem computes exactly what it would alone, up to
@@ -33,7 +34,7 @@ __global__ __launch_bounds__(BS) void kb_fold(const BSeg* __restrict__ segs)
 
 #define BSEG()                                                \
     const int64_t b_ = xcd_block(blockIdx.x, gridDim.x);      \
-    const BSeg& S = segs[find_seg(segs, nseg, b_)];           \
+    const BSeg& S = segs[seg_map(segs, nseg)[b_]];            \
     if(gated(S.gate)) return;                                 \
     const int64_t lb = b_ - S.blk0;
 
@@ -437,10 +438,18 @@ struct Planner
         next_rows = 1;
         if(!segs.empty())
         {
+            if(segs.size() > MAX_SEGS)
+            {
+                set_error("too many problems in one batched launch");
+                return -2;
+            }
+            std::vector<char> h(table_bytes(segs.size(), nb));
+            std::memcpy(h.data(), segs.data(), sizeof(BSeg) * segs.size());
+            block_map(segs.data(), segs.size(), reinterpret_cast<uint16_t*>(h.data() + sizeof(BSeg) * segs.size()));
             BSeg* d = nullptr;
-            HPCGX_HIP(hipMalloc(&d, sizeof(BSeg) * segs.size()));
-            HPCGX_HIP(hipMemcpy(d, segs.data(), sizeof(BSeg) * segs.size(), hipMemcpyHostToDevice));
+            HPCGX_HIP(hipMalloc(&d, h.size()));
             B.tables.push_back(d);
+            HPCGX_HIP(hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice));
             L.segs = d;
             out->push_back(L);
         }

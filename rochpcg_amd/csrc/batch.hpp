@@ -35,21 +35,23 @@ struct BSeg
     RedArgs ra;        // the problem's reduction scratch and scalars
 };
 
-// The segment of launch-relative (logical) block b: the last segment whose
-// first block is <= b.  Segments are numbered in block order (blk0 strictly
-// increasing, every nblk >= 1), so this is the segment holding b.
-__host__ __device__ inline int find_seg(const BSeg* __restrict__ segs, int nseg, int64_t b)
+// A launch's device table is its nseg BSeg records followed by the segment of
+// every logical block, one uint16 each (map[b] = j for b in [blk0_j, blk0_j +
+// nblk_j)), so a block finds its segment with one load instead of a search.
+constexpr size_t MAX_SEGS = 65535;
+
+__host__ __device__ inline const uint16_t* seg_map(const BSeg* segs, int nseg)
 {
-    int lo = 0, hi = nseg - 1;
-    while(lo < hi)
-    {
-        const int mid = (lo + hi + 1) >> 1;
-        if(segs[mid].blk0 <= b)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
+    return reinterpret_cast<const uint16_t*>(segs + nseg);
+}
+
+inline size_t table_bytes(size_t nseg, int64_t nblocks) { return sizeof(BSeg) * nseg + 2 * (size_t)nblocks; }
+
+// fill the block map of segments whose blk0/nblk assign_blocks has set
+inline void block_map(const BSeg* segs, size_t nseg, uint16_t* map)
+{
+    for(size_t j = 0; j < nseg; ++j)
+        for(int64_t b = segs[j].blk0; b < segs[j].blk0 + segs[j].nblk; ++b) map[b] = (uint16_t)j;
 }
 
 // Blocks of the segments of one launch, in order: a vector launch gives each

@@ -6,13 +6,14 @@
 //
 // stdin: per launch a line "block vector_kind rows nseg" followed by nseg
 // lines "off size".  For every launch it checks that
-//   - find_seg maps every logical block to the segment holding it;
+//   - the block map sends every logical block to the segment holding it;
 //   - xcd_block is a permutation of the launch's blocks;
 //   - every row (element) of every segment is visited by exactly one thread
 //     (R consecutive rows per thread in row launches, chunks in vector ones);
 //   - two rows per thread are used only where every offset and size is even.
 // Prints "ok <launches> <blocks>" or the first failure.
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "../../rochpcg_amd/csrc/batch.hpp"
@@ -43,7 +44,11 @@ int main()
             return 1;
         }
         const int64_t nb = assign_blocks(segs.data(), segs.size(), block, vector_kind != 0, rows);
-        // segment search against a linear scan; the XCD order is a permutation
+        // the block map against a linear scan; the XCD order is a permutation
+        std::vector<char> table(table_bytes(segs.size(), nb));
+        std::memcpy(table.data(), segs.data(), sizeof(BSeg) * segs.size());
+        block_map(segs.data(), segs.size(), reinterpret_cast<uint16_t*>(table.data() + sizeof(BSeg) * segs.size()));
+        const uint16_t* map = seg_map(reinterpret_cast<const BSeg*>(table.data()), nseg);
         std::vector<char> hit((size_t)nb, 0);
         for(int64_t bi = 0; bi < nb; ++bi)
         {
@@ -57,10 +62,10 @@ int main()
             int lin = -1;
             for(int j = 0; j < nseg; ++j)
                 if(segs[(size_t)j].blk0 <= b && b < segs[(size_t)j].blk0 + segs[(size_t)j].nblk) lin = j;
-            const int j = find_seg(segs.data(), nseg, b);
+            const int j = map[b];
             if(j != lin)
             {
-                printf("FAIL launch %ld: block %lld in segment %d, search gave %d\n", launches, (long long)b, lin, j);
+                printf("FAIL launch %ld: block %lld in segment %d, the map gives %d\n", launches, (long long)b, lin, j);
                 return 1;
             }
         }
