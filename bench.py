@@ -48,6 +48,16 @@ def spmv_bytes(m, vb=8):
     return m * (27 * (4 + vb) + 8 + 8)
 
 
+def vcycle_compulsory_bytes(level_rows):
+    """FP64 V-cycle bytes by the compulsory-traffic model of the problem
+    analysis (full-row ELL passes, x reused perfectly): per row of each level
+    above the coarsest, 364 B zero-guess SymGS + 43 B fused residual and
+    restriction + 4 B prolongation + 667.5 B post-smoothing SymGS; per row of
+    the coarsest level, 364 B."""
+    *upper, coarsest = level_rows
+    return 1078.5 * sum(upper) + 364.0 * coarsest
+
+
 def cpu_baseline(dims, iters, threads):
     """CG_ref on the host, natural order, b = A*ones, x0 = 0: the reference's own
     _ref code (CG_ref.cpp, ComputeMG_ref.cpp, ComputeSYMGS_ref.cpp, ... built
@@ -283,6 +293,9 @@ def main():
     spmv_gbps = spmv_bytes(G.m) / (spmv_ms * 1e-3) / 1e9
     mg_ms = ph["mg"] / K
     mg_gbps = mg64 / (mg_ms * 1e-3) / 1e9
+    # the same time against the full-row compulsory byte model (the sweeps here
+    # read only the slot windows they need, so they move fewer bytes than it counts)
+    mg_comp = vcycle_compulsory_bytes([L.m for L in G.levels])
     log(f"phases (ms per iteration): " + ", ".join(f"{k} {v / K:.3f}" for k, v in ph.items()))
     # the box's read ceiling for the matrix stream the SpMV reads (the level-0
     # column and value arrays, 16-byte loads, no gathers), in the same run, and
@@ -339,7 +352,9 @@ def main():
         "mg_roofline": {"kernels": "FP64 V-cycle, all levels (sweeps, restriction, prolongation)",
                         "bound": "hbm", "achieved": round(mg_gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                         "frac": round(mg_gbps / HBM_PEAK_GBPS, 4), "bytes_per_vcycle": int(mg64),
-                        "ms_per_vcycle": round(mg_ms, 4)},
+                        "ms_per_vcycle": round(mg_ms, 4),
+                        "compulsory_bytes_per_vcycle": int(mg_comp),
+                        "frac_compulsory_model": round(mg_comp / (mg_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)},
     }
 
     modes = (("f64", hip.CG_F64, 0), ("mixed_refine10", hip.CG_MIXED, 10), ("mixed", hip.CG_MIXED, 0))

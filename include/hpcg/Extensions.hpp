@@ -24,6 +24,19 @@ int HPCGX_Init(int device);
 int HPCGX_ReferenceCG(const SparseMatrix& A, int max_iter, double tolerance, int& niters, double& normr,
                       double& normr0, double* times);
 
+// CheckProblem's statements (CheckProblem.cpp:78-179) checked on the device
+// over every row of A's level in its optimized form: the permutation, the
+// colouring, each ELL row's entries (exactly the in-range 27-point neighbours
+// in ascending order, 26 / -1, the closed-form count), the diagonal arrays and
+// row descriptors, the level's nonzero total, and -- when b, x and xexact are
+// all given -- b = 26 - (nnz_row - 1), x = 0, xexact = 1.  Returns the number
+// of mismatches (0: the problem is as generated), or -1 when A is not
+// optimized or a vector does not match it.  counts (may be null) receives 8
+// entries: mismatches in the permutation, colouring, ELL rows, values,
+// descriptors and vectors, then the stored nonzeros and the rows checked.
+long long HPCGX_CheckProblem(const SparseMatrix& A, const Vector* b, const Vector* x, const Vector* xexact,
+                             long long* counts);
+
 // TestCG's exaggeration of the diagonal and of b (TestCG.cpp:67-95): the entry
 // at position i is scaled by (g+2)*1e6 when its global row id g < 9 and by 1e6
 // otherwise.  Like the reference, g is the natural row id of position i

@@ -5,6 +5,7 @@
 #include <iostream>
 #include "CG.hpp"
 #include "CGData.hpp"
+#include "CheckProblem.hpp"
 #include "ComputeDotProduct.hpp"
 #include "ComputeMG.hpp"
 #include "ComputeProlongation.hpp"

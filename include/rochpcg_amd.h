@@ -85,6 +85,18 @@ int hpcgx_level_arrays(const hpcgx_problem* p, int level, void** ptrs);
 /* device pointers of the CG data and system: ptrs[0..7] = b, x, xexact, r,
  * z (f64, or NULL), p, Ap, z (f32 mixed-mode, or NULL)  (CGData.hpp:55-61) */
 int hpcgx_vectors(const hpcgx_problem* p, void** ptrs);
+/* CheckProblem (CheckProblem.cpp:78-179, run on every level by main.cpp:236-246)
+ * on the device, on the optimized form of a level: counts[0..7] = errors in
+ * the permutation (not a bijection), the colouring (not proper, colour table
+ * or natural order inside a colour wrong), the ELL rows (not exactly the
+ * row's in-range 27-point neighbours through perm, in strictly ascending
+ * order, split-padded, with the closed-form count), the values (26 / -1 / 0
+ * padding in every store, the diagonal arrays), the row descriptors and
+ * diag_idx, and -- when check_vectors is set and level is 0 -- b = 26 -
+ * (nnz_row - 1), x = 0, xexact = 1; then the stored nonzeros and the rows
+ * checked.  Every row is checked, at any size.  Valid for the matrix as
+ * generated (HIPReplaceMatrixDiagonal or TestCG's scaling change the values). */
+int hpcgx_check_problem(hpcgx_problem* p, int level, int check_vectors, int64_t* counts);
 /* refill b / xexact / x: kind 0 = the reference's b = A*ones, xexact = 1
  * (GenerateProblem.cpp:300); kind 1 = xexact uniform in [-1,1] from
  * splitmix64(seed), b = A*xexact summed in natural order; x = 0 */

@@ -54,6 +54,7 @@ def _load():
         "hpcgx_level_arrays": ([vp, i32, P(vp)], i32),
         "hpcgx_vectors": ([vp, P(vp)], i32),
         "hpcgx_make_rhs": ([vp, i32, C.c_uint64], i32),
+        "hpcgx_check_problem": ([vp, i32, i32, P(i64)], i32),
         "hpcgx_spmv": ([vp, i32, vp, vp], i32),
         "hpcgx_symgs": ([vp, i32, vp, vp], i32),
         "hpcgx_symgs_zero": ([vp, i32, vp, vp], i32),
@@ -326,6 +327,15 @@ class Problem:
 
     def make_rhs(self, kind=0, seed=0):
         _check(lib.hpcgx_make_rhs(self.h, kind, seed))
+
+    CHECK_FIELDS = ("perm", "color", "ell", "value", "desc", "vectors", "nnz", "rows")
+
+    def check_problem(self, level=0, vectors=False):
+        """CheckProblem on the device (hpcgx_check_problem): error counts per
+        kind, the stored nonzeros and the rows checked."""
+        counts = (C.c_int64 * 8)()
+        _check(lib.hpcgx_check_problem(self.h, level, int(vectors), counts))
+        return dict(zip(self.CHECK_FIELDS, (int(v) for v in counts)))
 
     # -- operators (reference names in the C ABI docs) -------------------------
     def spmv(self, x, y, level=0):

@@ -391,6 +391,7 @@ int main(int argc, char** argv)
     printf("\nSetup Phase took %0.2lf sec\n", times[9]);
 
     // ---- optimization (main.cpp:350-361) ---------------------------------
+    int global_failure = 0;
     CGData data;
     t = now();
     if(OptimizeProblem(A, data, b, x, xexact))
@@ -400,6 +401,36 @@ int main(int argc, char** argv)
     }
     times[7] = now() - t;
     printf("\nOptimization Phase took %0.2lf sec\n", times[7]);
+
+    // ---- assembled-data check (main.cpp:236-246): CheckProblem on every
+    // level, b / x / xexact on the finest; the matrices exist once
+    // OptimizeProblem has built them, so the check runs here, on the device
+    long long check_bad = 0, check_rows = 0, check_nnz = 0;
+    if(prm.verify)
+    {
+        printf("\nChecking assembled data ...\n");
+        SparseMatrix* cm = &A;
+        for(int l = 0; l < levels && cm; ++l, cm = cm->Ac)
+        {
+            long long c[8] = {};
+            const long long bad = l == 0 ? HPCGX_CheckProblem(*cm, &b, &x, &xexact, c)
+                                         : HPCGX_CheckProblem(*cm, nullptr, nullptr, nullptr, c);
+            check_bad += bad < 0 ? 1 : bad;
+            check_rows += c[7];
+            check_nnz += c[6];
+            if(bad != 0)
+                printf("CheckProblem level %d: %lld mismatches (perm %lld, colouring %lld, ELL %lld, values %lld, "
+                       "descriptors %lld, vectors %lld)\n",
+                       l, bad, c[0], c[1], c[2], c[3], c[4], c[5]);
+        }
+        printf("CheckProblem: %lld rows, %lld nonzeros on %d levels, %lld mismatches\n", check_rows, check_nnz,
+               levels, check_bad);
+        if(check_bad)
+        {
+            global_failure = 1;
+            if(log) fprintf(log, "CheckProblem found %lld mismatches in the assembled data.\n", check_bad);
+        }
+    }
 
     // ---- SpMV+MG timing (main.cpp:284-305, on this library's kernels) ----
     const int ncalls = prm.quick ? 1 : 10;
@@ -428,7 +459,6 @@ int main(int argc, char** argv)
     double ref_tol = prm.tol;
     int niters = 0;
     double normr = 0.0, normr0 = 0.0;
-    int global_failure = 0;
     if(prm.verify)
     {
         // CG_ref in natural order with the reference's unoptimized operators
@@ -571,6 +601,11 @@ int main(int argc, char** argv)
     Yaml& pr = sc.add("Preconditioned");
     pr.add("Maximum iteration count", tcg.max_prec);
     pr.add("Expected iteration count", 2);
+    Yaml& ad = doc.add("Assembled Data Check");
+    ad.add("Result", prm.verify ? (check_bad == 0 ? "PASSED" : "FAILED") : "SKIPPED (--tol given)");
+    ad.add("Rows checked", check_rows);
+    ad.add("Nonzeros checked", check_nnz);
+    ad.add("Mismatches", check_bad);
     Yaml& ds = doc.add("Departure from Symmetry |x'Ay-y'Ax|/(2*||x||*||A||*||y||)/epsilon");
     ds.add("Result", prm.verify ? (tsym.fail == 0 ? "PASSED" : "FAILED") : "SKIPPED (--tol given)");
     ds.add("Departure for SpMV", tsym.spmv);

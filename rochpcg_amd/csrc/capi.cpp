@@ -161,6 +161,17 @@ int hpcgx_vectors(const hpcgx_problem* p, void** ptrs)
     return 0;
 }
 
+int hpcgx_check_problem(hpcgx_problem* p, int level, int check_vectors, int64_t* counts)
+{
+    CHECK_P(p);
+    CHECK_LEVEL(p, level);
+    CHECK_PTRS(counts);
+    Problem* P = p->P;
+    const bool vec = check_vectors != 0 && level == 0;
+    return check_problem(P->L[level], vec ? P->b : nullptr, vec ? P->x : nullptr, vec ? P->xexact : nullptr, counts,
+                         P->s);
+}
+
 int hpcgx_make_rhs(hpcgx_problem* p, int kind, uint64_t seed)
 {
     CHECK_P(p);

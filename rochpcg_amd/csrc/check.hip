@@ -1,0 +1,247 @@
+// check.hip -- CheckProblem on the device, for the optimized (multicolour,
+// sliced-ELL) form of a level.
+//
+// The reference's benchmark driver copies the assembled problem to the host
+// and runs CheckProblem on every level (main.cpp:236-246): every row of the
+// 27-point operator holds exactly its in-range neighbours, in order, with
+// value 26 on the diagonal and -1 elsewhere; the row's nonzero count and the
+// level's total are right; b = 26 - (nnz_row - 1), x = 0 and xexact = 1 on the
+// finest level (CheckProblem.cpp:78-179).  Here the same statements are
+// checked on the device, one thread per row, for all rows at any size, on the
+// form the solver runs:
+//   - perm is a bijection of the rows; the colours form a proper colouring;
+//     colour c owns rows [offsets[c], offsets[c+1]) and keeps the natural
+//     order inside it;
+//   - multicolour row r = perm[i] holds exactly the neighbours of natural row
+//     i, through perm, in strictly ascending column order, split-padded
+//     (lower entries and the diagonal from slot 0, upper entries ending at
+//     slot 26, -1 in between), with the row's nonzero count from the closed
+//     form;
+//   - every store's values (26 on the diagonal, -1 off it, 0 in padding), the
+//     diagonal arrays, diag_idx and the row descriptors agree with it;
+//   - optionally b, x and xexact of the default right-hand side.
+// The neighbour set is established without re-running the builder's sort:
+// every stored column maps back through the inverse permutation to a grid
+// point within one step of the row's, the entries are distinct (strictly
+// ascending), and their count is the closed-form count -- so they are exactly
+// the neighbours.
+#include "ops.hpp"
+#include "setup.hpp"
+
+namespace hpcgx
+{
+
+struct CheckLevel
+{
+    int nx, ny, nz;
+    int64_t m;
+    int ncolors;
+    int offsets[MAX_COLORS + 1];
+    const int32_t* perm;
+    const int32_t* colors;
+    const int32_t* col;
+    const double* val;      // may be null
+    const __half* val_h;    // may be null
+    const int32_t* diag_idx;
+    const int32_t* desc;
+    const double* inv_diag; // may be null
+    const double* diag64;   // may be null
+    const __half* diag_h;   // may be null
+    const double* b;        // all three null: vectors not checked
+    const double* x;
+    const double* xexact;
+};
+
+__global__ __launch_bounds__(256) void k_fill_neg(int64_t n, int32_t* __restrict__ a)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if(i < n) a[i] = -1;
+}
+
+// iperm[perm[i]] = i; perm entries out of range are counted
+__global__ __launch_bounds__(256) void k_invert(CheckLevel L, int32_t* __restrict__ iperm,
+                                                unsigned long long* __restrict__ err)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if(i >= L.m) return;
+    const int32_t r = L.perm[i];
+    if(r < 0 || r >= L.m)
+        atomicAdd(err + CHECK_PERM, 1ull);
+    else
+        iperm[r] = (int32_t)i;
+}
+
+__device__ __forceinline__ void coords(const CheckLevel& L, int64_t i, int& ix, int& iy, int& iz)
+{
+    const int64_t nxy = (int64_t)L.nx * L.ny;
+    iz = (int)(i / nxy);
+    const int64_t rem = i - (int64_t)iz * nxy;
+    iy = (int)(rem / L.nx);
+    ix = (int)(rem - (int64_t)iy * L.nx);
+}
+
+template <int BS>
+__device__ __forceinline__ unsigned long long block_total(unsigned long long v, unsigned long long* lds)
+{
+#pragma unroll
+    for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned long long s = 0;
+    if(threadIdx.x == 0)
+        for(int w = 0; w < BS / 64; ++w) s += lds[w];
+    __syncthreads();
+    return s;
+}
+
+// one multicolour row per thread; per-block totals of each error kind and of
+// the stored nonzeros
+__global__ __launch_bounds__(256) void k_check_rows(CheckLevel L, const int32_t* __restrict__ iperm,
+                                                    unsigned long long* __restrict__ err)
+{
+    __shared__ unsigned long long lds[4];
+    unsigned long long e[CHECK_KINDS] = {};
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if(r < L.m)
+    {
+        const int32_t i = iperm[r];
+        if(i < 0 || L.perm[i] != r)
+            e[CHECK_PERM]++;
+        else
+        {
+            int c = 0;
+            while(c + 1 < L.ncolors && r >= L.offsets[c + 1]) ++c;
+            const int mycol = L.colors[i];
+            if(mycol != c || r < L.offsets[c] || r >= L.offsets[c + 1]) e[CHECK_COLOR]++;
+            if(r > L.offsets[c] && iperm[r - 1] >= i) e[CHECK_COLOR]++;   // natural order inside a colour
+            int ix, iy, iz;
+            coords(L, i, ix, iy, iz);
+            const int nnz_row = (1 + (ix > 0) + (ix < L.nx - 1)) * (1 + (iy > 0) + (iy < L.ny - 1)) *
+                                (1 + (iz > 0) + (iz < L.nz - 1));
+            int32_t cc[W];
+#pragma unroll
+            for(int p = 0; p < W; ++p) cc[p] = L.col[ell_at(p, r)];
+            int ne = 0, d = -1;
+            int32_t prev = -1;
+#pragma unroll
+            for(int p = 0; p < W; ++p)
+            {
+                if(cc[p] < 0) continue;
+                ++ne;
+                if(cc[p] <= prev) e[CHECK_ELL]++;   // strictly ascending (hence distinct)
+                prev = cc[p];
+                if(cc[p] == r) d = (d < 0) ? p : 99;
+            }
+            const int nup = ne - d - 1;
+            const int us = W - nup;
+            if(ne != nnz_row || d < 0 || d >= W) e[CHECK_ELL]++;
+            const int last_off = L.offsets[L.ncolors - 1];
+            int tl = W;
+            if(d >= 0 && d < W)
+            {
+#pragma unroll
+                for(int p = 0; p < W; ++p)
+                {
+                    // split padding: [0, d] and [us, W) valid, -1 in between
+                    const bool should = (p <= d) || (p >= us);
+                    if(should != (cc[p] >= 0)) e[CHECK_ELL]++;
+                    if(cc[p] < 0) continue;
+                    if(cc[p] >= L.m)
+                    {
+                        e[CHECK_ELL]++;
+                        continue;
+                    }
+                    const int32_t j = iperm[cc[p]];
+                    if(j < 0)
+                    {
+                        e[CHECK_ELL]++;
+                        continue;
+                    }
+                    int jx, jy, jz;
+                    coords(L, j, jx, jy, jz);
+                    const int dx = jx - ix, dy = jy - iy, dz = jz - iz;
+                    if(dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) e[CHECK_ELL]++;
+                    if(p != d && L.colors[j] == mycol) e[CHECK_COLOR]++;   // proper colouring
+                    if(p != d && cc[p] >= last_off && tl == W) tl = p;
+                }
+                // values of every store
+#pragma unroll
+                for(int p = 0; p < W; ++p)
+                {
+                    const double want = (cc[p] < 0) ? 0.0 : (p == d ? 26.0 : -1.0);
+                    if(L.val && L.val[ell_at(p, r)] != want) e[CHECK_VALUE]++;
+                    if(L.val_h && (double)__half2float(L.val_h[ell_at(p, r)]) != want) e[CHECK_VALUE]++;
+                }
+                if(L.diag64 && L.diag64[r] != 26.0) e[CHECK_VALUE]++;
+                if(L.inv_diag && L.inv_diag[r] != 1.0 / 26.0) e[CHECK_VALUE]++;
+                if(L.diag_h && __half2float(L.diag_h[r]) != 26.0f) e[CHECK_VALUE]++;
+                const int32_t ds = L.desc[r];
+                if(L.diag_idx[r] != d || (ds & 255) != d || ((ds >> 8) & 255) != tl || ((ds >> 16) & 255) != us)
+                    e[CHECK_DESC]++;
+            }
+            if(L.b)
+            {
+                if(L.b[r] != 26.0 - (double)(nnz_row - 1)) e[CHECK_VECTORS]++;
+                if(L.x[r] != 0.0) e[CHECK_VECTORS]++;
+                if(L.xexact[r] != 1.0) e[CHECK_VECTORS]++;
+            }
+            e[CHECK_NNZ] = (unsigned long long)ne;
+        }
+        e[CHECK_ROWS] = 1;
+    }
+#pragma unroll
+    for(int k = 0; k < CHECK_KINDS; ++k)
+    {
+        const unsigned long long s = block_total<256>(e[k], lds);
+        if(threadIdx.x == 0 && s) atomicAdd(err + k, s);
+    }
+}
+
+int check_problem(const Level& L, const double* b, const double* x, const double* xexact, int64_t* counts,
+                  hipStream_t s)
+{
+    if(L.ncolors < 1 || L.ncolors > MAX_COLORS || !L.perm || !L.colors || !L.ell_col || !L.desc || !L.diag_idx)
+    {
+        set_error("level not built");
+        return -2;
+    }
+    CheckLevel C;
+    C.nx = L.nx;
+    C.ny = L.ny;
+    C.nz = L.nz;
+    C.m = L.m;
+    C.ncolors = L.ncolors;
+    for(int c = 0; c <= L.ncolors; ++c) C.offsets[c] = L.offsets[c];
+    C.perm = L.perm;
+    C.colors = L.colors;
+    C.col = L.ell_col;
+    C.val = L.ell_val;
+    C.val_h = L.ell_h;
+    C.diag_idx = L.diag_idx;
+    C.desc = L.desc;
+    C.inv_diag = L.inv_diag;
+    C.diag64 = L.diag64;
+    C.diag_h = L.diag_h;
+    C.b = b;
+    C.x = x;
+    C.xexact = xexact;
+    DevTemp iperm, err;
+    HPCGX_HIP(hipMalloc(&iperm.p, sizeof(int32_t) * L.m));
+    HPCGX_HIP(hipMalloc(&err.p, sizeof(unsigned long long) * CHECK_KINDS));
+    HPCGX_HIP(hipMemsetAsync(err.p, 0, sizeof(unsigned long long) * CHECK_KINDS, s));
+    const unsigned g = (unsigned)ceil_div(L.m, 256);
+    hipLaunchKernelGGL(k_fill_neg, dim3(g), dim3(256), 0, s, L.m, (int32_t*)iperm.p);
+    HPCGX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_invert, dim3(g), dim3(256), 0, s, C, (int32_t*)iperm.p, (unsigned long long*)err.p);
+    HPCGX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_check_rows, dim3(g), dim3(256), 0, s, C, (const int32_t*)iperm.p,
+                       (unsigned long long*)err.p);
+    HPCGX_HIP(hipGetLastError());
+    unsigned long long h[CHECK_KINDS];
+    HPCGX_HIP(hipMemcpyAsync(h, err.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    HPCGX_HIP(hipStreamSynchronize(s));
+    for(int k = 0; k < CHECK_KINDS; ++k) counts[k] = (int64_t)h[k];
+    return 0;
+}
+
+}  // namespace hpcgx

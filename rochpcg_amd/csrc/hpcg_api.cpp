@@ -677,6 +677,46 @@ int HPCGX_ReferenceCG(const SparseMatrix& A, int max_iter, double tolerance, int
     return 0;
 }
 
+long long HPCGX_CheckProblem(const SparseMatrix& A, const Vector* b, const Vector* x, const Vector* xexact,
+                             long long* counts)
+{
+    Problem* P;
+    int l;
+    if(!level_of(A, P, l)) return -1;
+    const Level& L = P->L[l];
+    for(const Vector* v : {b, x, xexact})
+        if(v && (!v->d_values || v->localLength != L.m))
+        {
+            set_error("CheckProblem: a vector does not match the matrix");
+            return -1;
+        }
+    const bool vec = b && x && xexact;
+    int64_t c[CHECK_KINDS];
+    if(check_problem(L, vec ? b->d_values : nullptr, vec ? x->d_values : nullptr, vec ? xexact->d_values : nullptr,
+                     c, P->s))
+        return -1;
+    long long bad = 0;
+    for(int k = 0; k < CHECK_NNZ; ++k) bad += c[k];
+    if(c[CHECK_NNZ] != L.nnz || c[CHECK_ROWS] != L.m) ++bad;
+    if(counts)
+        for(int k = 0; k < CHECK_KINDS; ++k) counts[k] = c[k];
+    return bad;
+}
+
+void CheckProblem(SparseMatrix& A, Vector* b, Vector* x, Vector* xexact)
+{
+    long long c[CHECK_KINDS] = {};
+    const long long bad = HPCGX_CheckProblem(A, b, x, xexact, c);
+    if(bad < 0)
+        fprintf(stderr, "CheckProblem: %s\n", last_error());
+    else if(bad > 0)
+        fprintf(stderr,
+                "CheckProblem: %lld mismatches (perm %lld, colouring %lld, ELL rows %lld, values %lld, "
+                "descriptors %lld, vectors %lld; %lld nonzeros over %lld rows)\n",
+                bad, c[CHECK_PERM], c[CHECK_COLOR], c[CHECK_ELL], c[CHECK_VALUE], c[CHECK_DESC], c[CHECK_VECTORS],
+                c[CHECK_NNZ], c[CHECK_ROWS]);
+}
+
 int HPCGX_TestCGScale(const SparseMatrix& A, Vector& diagonal, Vector& b)
 {
     Problem* P;

@@ -58,6 +58,24 @@ int launch_restrict(const Level& F, Prec prec, const void* r, bool r_f64, const 
 int launch_prolong(const Level& F, Prec prec, const void* xc, void* x, hipStream_t s, const int* gate);
 int launch_spmv_injected(const Level& F, const double* x, double* y, hipStream_t s);
 int launch_restrict_diff(const Level& F, const double* rf, const double* Axf, double* rc, hipStream_t s);
+// CheckProblem on the device (check.hip): counts[CHECK_*] of a level's
+// optimized form -- perm, colouring, ELL rows, values, descriptors and (when
+// b/x/xexact are given) the default right-hand side; counts[CHECK_NNZ] the
+// stored nonzeros, counts[CHECK_ROWS] the rows checked
+enum CheckKind : int
+{
+    CHECK_PERM = 0,
+    CHECK_COLOR = 1,
+    CHECK_ELL = 2,
+    CHECK_VALUE = 3,
+    CHECK_DESC = 4,
+    CHECK_VECTORS = 5,
+    CHECK_NNZ = 6,
+    CHECK_ROWS = 7,
+    CHECK_KINDS = 8
+};
+int check_problem(const Level& L, const double* b, const double* x, const double* xexact, int64_t* counts,
+                  hipStream_t s);
 // one read of a level's column and value arrays (16-byte non-temporal loads,
 // no gathers): the read ceiling of the matrix stream; sink holds >= min(4096,
 // sink_cap) doubles and is not normally written

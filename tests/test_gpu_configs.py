@@ -4,11 +4,13 @@
                    the same colour order (tests/golden/cg_history_320.npz),
                    1e-12 relative on every iteration (dot products summed with
                    compensation in the oracle).
-  320^3 mixed      x* uniform in [-1,1] (splitmix64 seed 1234), true residual
+  320^3 mixed      CheckProblem on the device over every row of every level;
+                   x* uniform in [-1,1] (splitmix64 seed 1234), true residual
                    every 10 iterations, solved to 1e-9 next to the FP64 solve.
   64 uneven        nx, ny, nz each from {16, 24, ..., 96} (splitmix64 seed 7),
                    problem k with x* from seed 7 + k, FP64 and mixed to 1e-9.
-  512^3            closed-form colours, permutation and ELL rows whose 64-bit
+  512^3            CheckProblem on the device over every row of every level;
+                   closed-form colours, permutation and ELL rows whose 64-bit
                    offsets exceed 2^31; x* from seed 99, FP64 and
                    mixed (true residual every 10 iterations) to 1e-9.
 
@@ -34,6 +36,16 @@ def _say(msg):
 def _natural(vec_mc, perm):
     """A multicolour vector in natural order: v_nat[i] = v_mc[perm[i]]."""
     return vec_mc[perm]
+
+
+def _check_problem_all_levels(G):
+    """CheckProblem on the device over every row of every level (vectors on
+    the finest, which must still hold the default right-hand side)."""
+    for lv, L in enumerate(G.levels):
+        c = G.check_problem(lv, vectors=(lv == 0))
+        _say(f"level {lv} ({L.nx}x{L.ny}x{L.nz}): {c}")
+        assert all(c[k] == 0 for k in ("perm", "color", "ell", "value", "desc", "vectors")), (lv, c)
+        assert c["rows"] == L.m and c["nnz"] == (3 * L.nx - 2) * (3 * L.ny - 2) * (3 * L.nz - 2), (lv, c)
 
 
 def _solution_checks(oracle_mod, G, perm, dims):
@@ -79,6 +91,8 @@ def test_config_mixed_320(hip, oracle_mod):
     iterations, against the FP64 solve of the same system."""
     dims = (320, 320, 320)
     G = hip.Problem(*dims, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
+    _say("config mixed 320^3: CheckProblem on every level")
+    _check_problem_all_levels(G)
     perm = G.level_views(0)["perm"].host()
     G.make_rhs(1, 1234)
     xs = oracle_mod.uniform_vector(1234, G.m) * 2.0 - 1.0
@@ -190,6 +204,8 @@ def test_config_512(hip, oracle_mod):
     _say("config 512^3: setup")
     G = hip.Problem(*dims, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
     m = G.m
+    _say("config 512^3: CheckProblem on every level")
+    _check_problem_all_levels(G)
     assert 16 * m >= 2 ** 31
     V = G.level_views(0)
     colors = V["colors"].host()
