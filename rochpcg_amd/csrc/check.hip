@@ -228,18 +228,35 @@ int check_problem(const Level& L, const double* b, const double* x, const double
     DevTemp iperm, err;
     HPCGX_HIP(hipMalloc(&iperm.p, sizeof(int32_t) * L.m));
     HPCGX_HIP(hipMalloc(&err.p, sizeof(unsigned long long) * CHECK_KINDS));
-    HPCGX_HIP(hipMemsetAsync(err.p, 0, sizeof(unsigned long long) * CHECK_KINDS, s));
+    // every step is attempted only while the previous ones succeeded, and the
+    // stream is drained before the temporaries are freed, on every path
+    unsigned long long h[CHECK_KINDS] = {};
     const unsigned g = (unsigned)ceil_div(L.m, 256);
-    hipLaunchKernelGGL(k_fill_neg, dim3(g), dim3(256), 0, s, L.m, (int32_t*)iperm.p);
-    HPCGX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_invert, dim3(g), dim3(256), 0, s, C, (int32_t*)iperm.p, (unsigned long long*)err.p);
-    HPCGX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_check_rows, dim3(g), dim3(256), 0, s, C, (const int32_t*)iperm.p,
-                       (unsigned long long*)err.p);
-    HPCGX_HIP(hipGetLastError());
-    unsigned long long h[CHECK_KINDS];
-    HPCGX_HIP(hipMemcpyAsync(h, err.p, sizeof(h), hipMemcpyDeviceToHost, s));
-    HPCGX_HIP(hipStreamSynchronize(s));
+    hipError_t e = hipMemsetAsync(err.p, 0, sizeof(unsigned long long) * CHECK_KINDS, s);
+    if(e == hipSuccess)
+    {
+        hipLaunchKernelGGL(k_fill_neg, dim3(g), dim3(256), 0, s, L.m, (int32_t*)iperm.p);
+        e = hipGetLastError();
+    }
+    if(e == hipSuccess)
+    {
+        hipLaunchKernelGGL(k_invert, dim3(g), dim3(256), 0, s, C, (int32_t*)iperm.p, (unsigned long long*)err.p);
+        e = hipGetLastError();
+    }
+    if(e == hipSuccess)
+    {
+        hipLaunchKernelGGL(k_check_rows, dim3(g), dim3(256), 0, s, C, (const int32_t*)iperm.p,
+                           (unsigned long long*)err.p);
+        e = hipGetLastError();
+    }
+    if(e == hipSuccess) e = hipMemcpyAsync(h, err.p, sizeof(h), hipMemcpyDeviceToHost, s);
+    const hipError_t es = hipStreamSynchronize(s);
+    if(e == hipSuccess) e = es;
+    if(e != hipSuccess)
+    {
+        set_error(std::string("CheckProblem on the device: ") + hipGetErrorString(e));
+        return -1;
+    }
     for(int k = 0; k < CHECK_KINDS; ++k) counts[k] = (int64_t)h[k];
     return 0;
 }
