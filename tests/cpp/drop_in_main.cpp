@@ -46,6 +46,23 @@ int main(int argc, char** argv)
     }
     printf("rows %d\nnonzeros %lld\nnblocks %d\nublocks %d\n", A.localNumberOfRows, A.totalNumberOfNonzeros, A.nblocks,
            A.ublocks);
+    // main.cpp:236-246: CheckProblem on every level, b / x / xexact on the
+    // finest (here on the optimized form, after OptimizeProblem)
+    {
+        long long bad = 0, rows = 0, nnz = 0;
+        SparseMatrix* cm = &A;
+        for(int level = 0; level < numberOfMgLevels; ++level, cm = cm->Ac)
+        {
+            CheckProblem(*cm, level == 0 ? &b : nullptr, level == 0 ? &x : nullptr, level == 0 ? &xexact : nullptr);
+            long long c[8] = {};
+            const long long e = level == 0 ? HPCGX_CheckProblem(*cm, &b, &x, &xexact, c)
+                                           : HPCGX_CheckProblem(*cm, nullptr, nullptr, nullptr, c);
+            bad += e < 0 ? 1 : e;
+            nnz += c[6];
+            rows += c[7];
+        }
+        printf("check_problem_mismatches %lld\ncheck_problem_rows %lld\ncheck_problem_nnz %lld\n", bad, rows, nnz);
+    }
     const local_int_t m = A.localNumberOfRows;
     bool opt = true;
     double t4 = 0.0;

@@ -33,6 +33,11 @@ def test_drop_in_layer(hip, oracle_mod, dims):
     assert v["done"] == 1
     assert v["rows"] == m and v["nonzeros"] == np.prod([3 * d - 2 for d in dims])
     assert v["nblocks"] == 8 and v["ublocks"] == 6
+    # CheckProblem on every level through the C++ layer
+    levels = [tuple(d >> k for d in dims) for k in range(4)]
+    assert v["check_problem_mismatches"] == 0
+    assert v["check_problem_rows"] == sum(np.prod(lv) for lv in levels)
+    assert v["check_problem_nnz"] == sum(np.prod([3 * d - 2 for d in lv]) for lv in levels)
     assert v["spmv_residual"] == 0.0            # integers in f64: A*ones == b exactly
     assert v["compute_residual_rc"] == 0 and v["compute_residual"] == 0.0
     assert v["l2g_identity"] == 1
@@ -125,6 +130,11 @@ def test_benchmark_driver_report(tmp_path, extra):
         sym = "Departure from Symmetry |x'Ay-y'Ax|/(2*||x||*||A||*||y||)/epsilon"
         assert y[f"{sym}/Result"] == "PASSED"
         assert opt == 50
+        # CheckProblem on every level (main.cpp:236-246): 32^3 + 16^3 + 8^3 + 4^3 rows
+        assert y["Assembled Data Check/Result"] == "PASSED"
+        assert y["Assembled Data Check/Mismatches"] == "0"
+        assert int(y["Assembled Data Check/Rows checked"]) == sum(n ** 3 for n in (32, 16, 8, 4))
+        assert int(y["Assembled Data Check/Nonzeros checked"]) == sum((3 * n - 2) ** 3 for n in (32, 16, 8, 4))
     for k in ("DDOT", "WAXPBY", "SpMV", "MG", "Total"):
         assert float(y[f"Benchmark Time Summary/{k}"]) > 0.0
     # the run log (init.cpp:243-249): one "Call [k] Scaled Residual [r]" line per timed set
