@@ -383,6 +383,11 @@ __device__ __forceinline__ void row_bwd_zero(int64_t row, const int32_t* __restr
                                              const int32_t* __restrict__ desc, XT* __restrict__ x,
                                              XT* __restrict__ u)
 {
+    // u = x*d - snap must use the rounded product that starts the running
+    // sum: left to the compiler, the product may or may not be contracted
+    // into the subtraction depending on the instruction schedule, and then
+    // two launch shapes of the same operator round u differently.
+#pragma clang fp contract(off)
     int32_t ds[R];
     ld<R>(desc + row, ds);
     const int start = W - NS;

@@ -100,8 +100,13 @@ def _yaml_fields(path):
 def test_benchmark_driver_report(tmp_path, extra):
     """xhpcg runs main.cpp's phases on the drop-in layer and writes the HPCG
     report: a valid 32^3 quick run, the V&V sections passed, 50 iterations per
-    set (a set is never shorter than the reference's 50; 32^3 reaches both its
-    own 50-iteration tolerance and 1e-9 within 50)."""
+    set with --tol=1e-9 (a set is never shorter than the reference's 50, and
+    1e-9 is reached well within 50).  With the default tolerance -- the scaled
+    residual of 50 natural-order CG_ref iterations, about 6.5e-21 at 32^3 --
+    the count is only bounded: that level is below rounding noise, where the
+    oracle's multicolour CG needs 49 iterations, its scaled residual rises
+    between iterations 50 and 51, and the reference's own CG_ref differs by
+    0.9 relative at iteration 50 between 1 and 8 OpenMP threads."""
     path = tmp_path / "report.yaml"
     if extra == ["hpcg.dat"]:
         # no dimensions on the command line: hpcg.dat in the working directory
@@ -129,7 +134,7 @@ def test_benchmark_driver_report(tmp_path, extra):
         assert int(y["Spectral Convergence Tests/Preconditioned/Maximum iteration count"]) <= 2
         sym = "Departure from Symmetry |x'Ay-y'Ax|/(2*||x||*||A||*||y||)/epsilon"
         assert y[f"{sym}/Result"] == "PASSED"
-        assert opt == 50
+        assert 50 <= opt <= 55
         # CheckProblem on every level (main.cpp:236-246): 32^3 + 16^3 + 8^3 + 4^3 rows
         assert y["Assembled Data Check/Result"] == "PASSED"
         assert y["Assembled Data Check/Mismatches"] == "0"
