@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_color_hist(int64_t m, int ncolors, cons
         if(lane == 0) cnt[wid][cc] = __popcll(b);
     }
     __syncthreads();
-    if(threadIdx.x < ncolors)
+    if((int)threadIdx.x < ncolors)
     {
         const int cc = threadIdx.x;
         hist[(int64_t)cc * nblk + blockIdx.x] = cnt[0][cc] + cnt[1][cc] + cnt[2][cc] + cnt[3][cc];
