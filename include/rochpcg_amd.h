@@ -62,7 +62,10 @@ int hpcgx_problem_destroy(hpcgx_problem* p);
 int hpcgx_level_info(const hpcgx_problem* p, int level, int64_t* info);
 /* colour table: sizes[c], offsets[c] (c < ncolors, offsets has ncolors+1
  * entries) and the per-JPL-round counts the reference records in A.sizes
- * (MultiColoring.cpp:287-306) */
+ * (MultiColoring.cpp:287-306), at most HPCGX_MAX_COLORS of them.  A level
+ * holds up to HPCGX_MAX_COLORS colours, the reference's MAX_COLORS
+ * (utils.hpp:48); a colouring that needs more fails problem creation. */
+#define HPCGX_MAX_COLORS 128
 int hpcgx_color_table(const hpcgx_problem* p, int level, int* sizes, int* offsets, int* round_sizes);
 /* device pointers of a level's arrays (SparseMatrix::ell_col_ind, ell_val,
  * diag_idx, inv_diag, perm; MGData::d_f2cOperator); any output may be NULL.

@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "librochpcg_amd.so")
 STORE_F64 = 1
 STORE_MIXED = 2
 ELL_SLICE = 128   # rows per slice of the sliced ELL (HPCGX_ELL_SLICE)
+MAX_COLORS = 128  # colours a level may hold (HPCGX_MAX_COLORS)
 CG_F64 = 0
 CG_MIXED = 1
 
@@ -277,7 +278,7 @@ class Problem:
         L = self.levels[level]
         sizes = np.zeros(L.ncolors, np.int32)
         offsets = np.zeros(L.ncolors + 1, np.int32)
-        rounds = np.zeros(64, np.int32)
+        rounds = np.zeros(MAX_COLORS, np.int32)
         _check(lib.hpcgx_color_table(self.h, level, sizes.ctypes.data, offsets.ctypes.data,
                                      rounds.ctypes.data))
         return sizes, offsets, rounds

@@ -15,7 +15,7 @@ namespace hpcgx
 {
 
 constexpr int W = 27;            // ELL width of the 27-point operator
-constexpr int MAX_COLORS = 32;   // colours a level may hold
+constexpr int MAX_COLORS = 128;  // colours a level may hold (the reference's MAX_COLORS, utils.hpp:48)
 constexpr int MAX_LEVELS = 8;
 constexpr int NUM_XCD = 8;       // XCDs of an MI355X
 

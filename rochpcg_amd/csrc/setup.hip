@@ -448,26 +448,29 @@ int color_level(Level& L, hipStream_t s)
     }
     const int32_t* colors = (const int32_t*)ta.p;
 
-    // per-colour counts via the block histogram; ncolors = highest colour + 1
+    // per-colour counts via the block histogram; ncolors = highest colour + 1.
+    // The colours drawn are below max(8, nblocks): rand() % 8 in the first
+    // rounds, then nblocks and nblocks + 1.
+    const int ncb = std::max(8, nblocks);
     const int64_t nblk = ceil_div(m, 256);
-    HPCGX_HIP(hipMalloc(&thist.p, sizeof(int32_t) * nblk * MAX_COLORS));
-    HPCGX_HIP(hipMalloc(&tbase.p, sizeof(int64_t) * nblk * MAX_COLORS));
+    HPCGX_HIP(hipMalloc(&thist.p, sizeof(int32_t) * nblk * ncb));
+    HPCGX_HIP(hipMalloc(&tbase.p, sizeof(int64_t) * nblk * ncb));
     int32_t* hist = (int32_t*)thist.p;
     int64_t* base = (int64_t*)tbase.p;
-    hipLaunchKernelGGL(k_color_hist, dim3((unsigned)nblk), dim3(256), 0, s, m, MAX_COLORS, colors, hist, nblk);
+    hipLaunchKernelGGL(k_color_hist, dim3((unsigned)nblk), dim3(256), 0, s, m, ncb, colors, hist, nblk);
     HPCGX_HIP(hipGetLastError());
-    std::vector<int32_t> hh((size_t)nblk * MAX_COLORS);
+    std::vector<int32_t> hh((size_t)nblk * ncb);
     HPCGX_HIP(hipMemcpyAsync(hh.data(), hist, sizeof(int32_t) * hh.size(), hipMemcpyDeviceToHost, s));
     HPCGX_HIP(hipStreamSynchronize(s));
     int64_t tot[MAX_COLORS] = {};
     int ncolors = 0;
-    for(int c = 0; c < MAX_COLORS; ++c)
+    for(int c = 0; c < ncb; ++c)
     {
         for(int64_t b = 0; b < nblk; ++b) tot[c] += hh[(size_t)c * nblk + b];
         if(tot[c] > 0) ncolors = c + 1;
     }
     L.ncolors = ncolors;
-    std::vector<int64_t> hb((size_t)nblk * MAX_COLORS);
+    std::vector<int64_t> hb((size_t)nblk * ncb);
     int64_t run = 0;
     for(int c = 0; c < ncolors; ++c)
     {

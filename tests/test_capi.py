@@ -21,6 +21,15 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_python_constants_match_the_header():
+    import rochpcg_amd
+    text = open(os.path.join(ROOT, "include", "rochpcg_amd.h")).read()
+    defines = dict(re.findall(r"#define HPCGX_(\w+)\s+(\d+)", text))
+    assert set(defines) >= {"STORE_F64", "STORE_MIXED", "ELL_SLICE", "CG_F64", "CG_MIXED", "MAX_COLORS"}
+    for name, value in defines.items():
+        assert getattr(rochpcg_amd, name) == int(value), name
+
+
 def test_errors_are_reported_not_fatal():
     import pytest
     import rochpcg_amd

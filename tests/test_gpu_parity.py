@@ -16,6 +16,9 @@ SHAPES = [(16, 16, 16), (32, 32, 32), (24, 16, 32), (32, 24, 16)]
 # medium grids: every level's setup and operators against the oracle between
 # the small grids above and the full-size configurations
 MEDIUM = [(96, 96, 96), (128, 64, 48)]
+# long thin grids: their coarsest levels (32x2x2, 64x2x2) need 38 and 70 JPL
+# colours, beyond the parity classes (the reference allows 128)
+ANISO = [(256, 16, 16), (512, 16, 16)]
 
 
 def _hip_problem(hip, dims, nlevels=4, stores=None):
@@ -28,7 +31,7 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize("dims", SHAPES + MEDIUM)
+@pytest.mark.parametrize("dims", SHAPES + MEDIUM + ANISO)
 def test_setup_bit_exact(hip, oracle_mod, dims):
     G = _hip_problem(hip, dims)
     O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
@@ -105,7 +108,7 @@ def test_setup_equal_hash_level(hip, oracle_mod):
     assert np.array_equal(np.take_along_axis(cols, keep, 1), o["cols"])
 
 
-@pytest.mark.parametrize("dims", SHAPES + MEDIUM)
+@pytest.mark.parametrize("dims", SHAPES + MEDIUM + ANISO)
 def test_operators_match_oracle(hip, oracle_mod, dims):
     """SpMV, SymGS, zero-guess SymGS on every level, the V-cycle, restriction
     and prolongation against the oracle.  32x24x16 ends on a 4x3x2 level whose
@@ -554,4 +557,41 @@ def test_small_and_odd_hierarchies_match_oracle(hip, oracle_mod, dims):
     res = G.cg(max_iter=20, tol=0.0, use_graph=True)
     ref = O.cg(O.rhs_ones(), max_iter=20)
     assert_history_close(res["history"], ref["history"])
+    G.close()
+
+
+@pytest.mark.parametrize("dims", ANISO)
+def test_many_colour_levels_cg(hip, oracle_mod, dims):
+    """A hierarchy whose coarsest level holds 38 or 70 colours: the colour
+    counts agree with the oracle's, and the 20-iteration CG history and the
+    mixed-precision solve's true residual hold their bars."""
+    G = _hip_problem(hip, dims)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    assert [L.ncolors for L in G.levels] == [L.ncolors for L in O.levels]
+    assert G.levels[3].ncolors > 32
+    res = G.cg(max_iter=20, tol=0.0, use_graph=True)
+    ref = O.cg(O.rhs_ones(), max_iter=20)
+    assert_history_close(res["history"], ref["history"])
+    G.vec["x"].upload(np.zeros(G.m))
+    mix = G.cg(max_iter=200, tol=1e-9, mode=hip.CG_MIXED, use_graph=True)
+    x = G.vec["x"].host()
+    true_res = np.linalg.norm(O.rhs_ones() - O.spmv(x)) / np.linalg.norm(O.rhs_ones())
+    assert mix["normr"] / mix["normr0"] <= 1e-9 and true_res <= 2e-9, (mix, true_res)
+    G.close()
+
+
+def test_more_colours_than_the_limit_are_refused(hip, oracle_mod):
+    """128x2x2 needs more than 128 JPL colours: creation fails with a message,
+    as the oracle's does (the reference's loop has no bound there and writes
+    past its MAX_COLORS-entry colour arrays, MultiColoring.cpp:257-310)."""
+    print("[gpu] problem (128, 2, 2) levels=1", flush=True)
+    with pytest.raises(hip.HPCGError, match="MAX_COLORS"):
+        hip.Problem(128, 2, 2, nlevels=1, stores=hip.STORE_F64)
+    with pytest.raises(ValueError):
+        oracle_mod.Problem(128, 2, 2, nlevels=1, order=oracle_mod.ORDER_JPL)
+    # the library is still usable afterwards
+    G = _hip_problem(hip, (32, 2, 2), nlevels=1, stores=hip.STORE_F64)
+    O = oracle_mod.Problem(32, 2, 2, nlevels=1, order=oracle_mod.ORDER_JPL)
+    assert G.levels[0].ncolors == O.levels[0].ncolors == 38
+    assert np.array_equal(G.level_arrays(0)["perm"], O.export(0)["perm"])
     G.close()
