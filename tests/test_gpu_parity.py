@@ -595,3 +595,58 @@ def test_more_colours_than_the_limit_are_refused(hip, oracle_mod):
     assert G.levels[0].ncolors == O.levels[0].ncolors == 38
     assert np.array_equal(G.level_arrays(0)["perm"], O.export(0)["perm"])
     G.close()
+
+
+@pytest.mark.parametrize("mode", ["f64", "mixed"])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_cg_edge_cases_match_oracle(hip, oracle_mod, mode, use_graph):
+    """CG.cpp's loop test `normr/normr0 > tolerance` (CG.cpp:148): a zero
+    right-hand side and an exact initial guess give 0/0 and no iteration,
+    with x left as it was; max_iter = 0 only forms the initial residual; a
+    loose tolerance stops at the oracle's iteration count."""
+    dims = (16, 16, 16)
+    G = _hip_problem(hip, dims)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    cg_mode = hip.CG_F64 if mode == "f64" else hip.CG_MIXED
+    m = G.m
+    b = O.rhs_ones()
+    db = hip.DeviceArray.from_host(b)
+
+    def run(b_dev, x0, **kw):
+        dx = hip.DeviceArray.from_host(x0)
+        res = G.cg(b=b_dev, x=dx, mode=cg_mode, use_graph=use_graph, **kw)
+        return res, dx.host()
+
+    res, x = run(hip.DeviceArray.from_host(np.zeros(m)), np.zeros(m), max_iter=50)
+    assert (res["niters"], res["normr"], res["normr0"]) == (0, 0.0, 0.0)
+    assert np.all(x == 0.0)
+    res, x = run(db, np.ones(m), max_iter=50)
+    assert (res["niters"], res["normr"], res["normr0"]) == (0, 0.0, 0.0)
+    assert np.all(x == 1.0)
+    ref = O.cg(b, max_iter=0)
+    res, x = run(db, np.zeros(m), max_iter=0)
+    assert res["niters"] == 0 and np.all(x == 0.0)
+    assert res["normr"] == res["normr0"] == pytest.approx(ref["normr0"], rel=1e-14)
+    for tol in (0.5, 1e-3):
+        ref = O.cg(b, max_iter=50, tol=tol)
+        res, x = run(db, np.zeros(m), max_iter=50, tol=tol)
+        if mode == "f64":
+            assert res["niters"] == ref["niters"], (tol, res["niters"], ref["niters"])
+        else:
+            assert abs(res["niters"] - ref["niters"]) <= 1, (tol, res["niters"], ref["niters"])
+        assert res["normr"] / res["normr0"] <= tol
+    G.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_unpreconditioned_cg_history(hip, oracle_mod, use_graph):
+    """CG without the V-cycle (CG.cpp:150-155, z = r): 30 iterations against
+    the oracle's, per iteration."""
+    dims = (32, 24, 16)
+    G = _hip_problem(hip, dims, stores=hip.STORE_F64)
+    O = oracle_mod.Problem(*dims, nlevels=4, order=oracle_mod.ORDER_JPL)
+    res = G.cg(max_iter=30, tol=0.0, precond=False, use_graph=use_graph)
+    ref = O.cg(O.rhs_ones(), max_iter=30, precond=False)
+    assert res["niters"] == 30
+    assert_history_close(res["history"], ref["history"])
+    G.close()
