@@ -29,7 +29,7 @@ def check_all_levels(G):
     return out
 
 
-@pytest.mark.parametrize("dims", [(16, 16, 16), (32, 24, 16), (24, 16, 32), (40, 8, 16), (8, 8, 8)])
+@pytest.mark.parametrize("dims", [(16, 16, 16), (32, 24, 16), (24, 16, 32), (40, 8, 16), (8, 8, 8), (256, 16, 16)])
 def test_check_problem_clean(hip, dims):
     print(f"[gpu] check_problem {dims}", flush=True)
     G = hip.Problem(*dims, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)

@@ -26,7 +26,7 @@ def run(*args):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dims", [(32, 32, 32), (24, 16, 32)])
+@pytest.mark.parametrize("dims", [(32, 32, 32), (24, 16, 32), (256, 16, 16)])
 def test_drop_in_layer(hip, oracle_mod, dims):
     v, _ = run(*dims)
     m = dims[0] * dims[1] * dims[2]
