@@ -7,20 +7,24 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 DIMS = [(16, 16, 16), (24, 16, 32), (32, 24, 16), (16, 32, 24), (40, 16, 16), (16, 16, 48)]
+# coarsest levels of 38, 8 and 22 colours: colour launches that only some
+# problems of the batch take part in
+MANY_COLOURS = [(256, 16, 16), (16, 16, 16), (128, 16, 16)]
 
 
 def _rhs_seed(k):
     return 7 + k
 
 
+@pytest.mark.parametrize("dims_list", [DIMS, MANY_COLOURS], ids=["uneven", "many_colours"])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_batched_equals_individual(hip, mode):
-    print(f"[gpu] batch of {len(DIMS)} problems, mode {mode}", flush=True)
-    B = hip.Batch(DIMS, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
+def test_batched_equals_individual(hip, mode, dims_list):
+    print(f"[gpu] batch of {len(dims_list)} problems, mode {mode}", flush=True)
+    B = hip.Batch(dims_list, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
     for k, P in enumerate(B.problems):
         P.make_rhs(1, _rhs_seed(k))
     res = B.cg(max_iter=300, tol=1e-9, mode=mode)
-    for k, dims in enumerate(DIMS):
+    for k, dims in enumerate(dims_list):
         P = B.problems[k]
         x = P.vec["x"].host()
         G = hip.Problem(*dims, nlevels=4, stores=hip.STORE_F64 | hip.STORE_MIXED)
